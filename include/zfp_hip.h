@@ -1,0 +1,102 @@
+/*
+ * zfp_hip.h -- C ABI of the MI355X (gfx950) zfp block codec.
+ *
+ * The high-level entry points take zfp's own structs (layouts of
+ * include/zfp.h, LLNL zfp 1.0.1) and are what an execution policy of the
+ * reference's zfp_compress()/zfp_decompress() dispatch table calls, the way
+ * the CUDA policy calls cuda_compress()/cuda_decompress()
+ * (src/cuda_zfp/cuZFP.h:9-10 via src/template/cudacompress.c:5-34).  The
+ * device-level entry points take plain pointers and sizes only.
+ *
+ * All entry points are synchronous with respect to their results unless a
+ * HIP stream argument says otherwise.  Errors are reported by return value;
+ * nothing is printed.
+ */
+#ifndef ZFP_HIP_H
+#define ZFP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "zfp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes of the device-level entry points */
+#define ZFP_HIP_OK            0
+#define ZFP_HIP_EINVAL        1  /* unsupported or inconsistent arguments */
+#define ZFP_HIP_ENODEV        2  /* no usable gfx950 device */
+#define ZFP_HIP_ENOMEM        3  /* device allocation failed */
+#define ZFP_HIP_ERUNTIME      4  /* a HIP call failed */
+#define ZFP_HIP_ENOINDEX      5  /* variable-rate stream without a block index */
+
+/*
+ * Replaces cuda_compress() (src/cuda_zfp/cuZFP.h:9, src/cuda_zfp/cuZFP.cu:357-414).
+ * Compresses `field` into the bit stream of `stream` starting at its current
+ * write position, including bits still buffered in the stream state (e.g. a
+ * header written by zfp_write_header).  Field data and stream buffer may be
+ * host or device memory (detected per pointer).  All modes except reversible
+ * are supported for 1D-4D int32, int64, float and double fields with any
+ * strides.  On success the stream is left flushed exactly as serial
+ * zfp_compress() leaves it and the return value is the stream size in bytes;
+ * on a request the policy declines, it returns 0 and leaves the stream
+ * untouched.
+ */
+size_t zfp_hip_compress(zfp_stream* stream, const zfp_field* field);
+
+/*
+ * Replaces cuda_decompress() (src/cuda_zfp/cuZFP.h:10, cuZFP.cu:416-...).
+ * Decompresses from the current read position and leaves the stream aligned
+ * as serial zfp_decompress() does.  Returns the stream size in bytes
+ * consumed so far (the value zfp_decompress() returns), or 0 when declined.
+ * Variable-rate streams are decoded with the block index recorded when the
+ * same buffer was compressed by zfp_hip_compress in this process, or with an
+ * index imported by zfp_hip_index_import; without one, block offsets are
+ * recovered by a sequential parse on the device.
+ */
+size_t zfp_hip_decompress(zfp_stream* stream, zfp_field* field);
+
+/*
+ * Device-level encode: no counterpart in the reference (its GPU policy only
+ * exposes the zfp_stream entry above).  Compresses a device-resident field
+ * (element 0 at d_data, extents n[0..3] with 0 for unused dimensions,
+ * strides s[0..3] in elements with 0 meaning the default) with the codec
+ * parameters (minbits, maxbits, maxprec, minexp) into the device-resident
+ * word array d_words, starting at bit `offset`.  Bits of d_words below
+ * `offset` are preserved; the stream is written up to the end of its last
+ * (zero-padded) word.  *bits_out receives the number of bits written.
+ * hip_stream is a hipStream_t (NULL: the default stream); the call returns
+ * after the work is complete.
+ */
+int zfp_hip_encode(int type, const size_t n[4], const ptrdiff_t s[4], const void* d_data,
+                   uint minbits, uint maxbits, uint maxprec, int minexp,
+                   uint64_t* d_words, uint64_t offset, uint64_t* bits_out,
+                   void* hip_stream);
+
+/* Device-level decode, the inverse of zfp_hip_encode.  *bits_out receives
+   the number of bits consumed. */
+int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4], void* d_data,
+                   uint minbits, uint maxbits, uint maxprec, int minexp,
+                   const uint64_t* d_words, uint64_t offset, uint64_t* bits_out,
+                   void* hip_stream);
+
+/* Number of per-block bit offsets recorded for the stream whose buffer
+   begins at `stream_begin` (0 if none); and export/import of that index as
+   64-bit absolute bit offsets, one per block plus the end offset, so that a
+   variable-rate stream written by this library can be decoded in parallel in
+   another process.  No counterpart in the reference. */
+size_t zfp_hip_index_size(const void* stream_begin);
+int zfp_hip_index_export(const void* stream_begin, uint64_t* offsets, size_t count);
+int zfp_hip_index_import(const void* stream_begin, const uint64_t* offsets, size_t count);
+void zfp_hip_index_clear(void);
+
+/* Library and device identification; 0 when a gfx950 device is usable. */
+int zfp_hip_device_check(void);
+const char* zfp_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

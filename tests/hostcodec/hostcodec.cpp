@@ -1,0 +1,135 @@
+// Host build of the codec's per-block device code (zfp_amd/csrc/codec_block.h)
+// for the CPU tests: the same encode_block/decode_block the gfx950 kernels run,
+// executed one block after another, so that their streams can be compared
+// with the oracle without a GPU.  Fixed-rate blocks go to b * maxbits;
+// variable-rate blocks are laid out by a length pass and a prefix sum, as on
+// the device.
+#include <cstdint>
+#include <cstddef>
+#include <cstdlib>
+#include <vector>
+
+#include "codec_block.h"
+
+using namespace zfphip;
+
+namespace {
+
+bool make_geom(int dims, const size_t* n, const ptrdiff_t* s, Geom& g, u32& nblocks)
+{
+  int64_t def[4] = {1, (int64_t)n[0], (int64_t)(n[0] * (n[1] ? n[1] : 1)),
+                    (int64_t)(n[0] * (n[1] ? n[1] : 1) * (n[2] ? n[2] : 1))};
+  uint64_t nb = 1;
+  for (int i = 0; i < 4; i++) {
+    if (i < dims) {
+      g.n[i] = (u32)n[i];
+      g.s[i] = (s && s[i]) ? s[i] : def[i];
+      g.nb[i] = (u32)((n[i] + 3) / 4);
+      nb *= g.nb[i];
+    }
+    else {
+      g.n[i] = 1; g.s[i] = 0; g.nb[i] = 1;
+    }
+  }
+  g.dims = dims;
+  nblocks = (u32)nb;
+  return true;
+}
+
+template <typename S, int D>
+uint64_t enc(const void* data, const size_t* n, const ptrdiff_t* s, const Params& z, uint64_t* words,
+             uint64_t offset)
+{
+  Geom g;
+  u32 nb;
+  make_geom(D, n, s, g, nb);
+  std::vector<uint64_t> off(nb + 1, 0);
+  for (u32 b = 0; b < nb; b++) {
+    u32 bc[4] = {0, 0, 0, 0};
+    block_coords<D>(b, g, bc);
+    BitCounter c;
+    c.pos = 0;
+    off[b + 1] = off[b] + encode_block<S, D>(c, (const S*)data, g, bc, z);
+  }
+  for (u32 b = 0; b < nb; b++) {
+    u32 bc[4] = {0, 0, 0, 0};
+    block_coords<D>(b, g, bc);
+    BitWriter w;
+    w.begin(words, offset + off[b], offset + off[b + 1]);
+    encode_block<S, D>(w, (const S*)data, g, bc, z);
+    w.finish();
+  }
+  return off[nb];
+}
+
+template <typename S, int D>
+uint64_t dec(void* data, const size_t* n, const ptrdiff_t* s, const Params& z, const uint64_t* words,
+             uint64_t nwords, uint64_t offset)
+{
+  Geom g;
+  u32 nb;
+  make_geom(D, n, s, g, nb);
+  uint64_t pos = offset;
+  for (u32 b = 0; b < nb; b++) {
+    u32 bc[4] = {0, 0, 0, 0};
+    block_coords<D>(b, g, bc);
+    BitReader r;
+    r.begin(words, nwords, pos);
+    pos += decode_block<S, D, true>(r, (S*)data, g, bc, z);
+  }
+  return pos - offset;
+}
+
+template <typename S>
+uint64_t enc_t(int dims, const void* data, const size_t* n, const ptrdiff_t* s, const Params& z,
+               uint64_t* words, uint64_t offset)
+{
+  switch (dims) {
+    case 1: return enc<S, 1>(data, n, s, z, words, offset);
+    case 2: return enc<S, 2>(data, n, s, z, words, offset);
+    case 3: return enc<S, 3>(data, n, s, z, words, offset);
+    default: return enc<S, 4>(data, n, s, z, words, offset);
+  }
+}
+
+template <typename S>
+uint64_t dec_t(int dims, void* data, const size_t* n, const ptrdiff_t* s, const Params& z,
+               const uint64_t* words, uint64_t nwords, uint64_t offset)
+{
+  switch (dims) {
+    case 1: return dec<S, 1>(data, n, s, z, words, nwords, offset);
+    case 2: return dec<S, 2>(data, n, s, z, words, nwords, offset);
+    case 3: return dec<S, 3>(data, n, s, z, words, nwords, offset);
+    default: return dec<S, 4>(data, n, s, z, words, nwords, offset);
+  }
+}
+
+} // namespace
+
+extern "C" uint64_t hostcodec_compress(int type, const size_t* n, const ptrdiff_t* s, const void* data,
+                                       unsigned minbits, unsigned maxbits, unsigned maxprec, int minexp,
+                                       uint64_t* words, uint64_t offset)
+{
+  int dims = n[0] ? n[1] ? n[2] ? n[3] ? 4 : 3 : 2 : 1 : 0;
+  Params z{minbits, maxbits, maxprec, minexp};
+  switch (type) {
+    case 1: return enc_t<i32>(dims, data, n, s, z, words, offset);
+    case 2: return enc_t<i64>(dims, data, n, s, z, words, offset);
+    case 3: return enc_t<float>(dims, data, n, s, z, words, offset);
+    default: return enc_t<double>(dims, data, n, s, z, words, offset);
+  }
+}
+
+extern "C" uint64_t hostcodec_decompress(int type, const size_t* n, const ptrdiff_t* s, void* data,
+                                         unsigned minbits, unsigned maxbits, unsigned maxprec, int minexp,
+                                         const uint64_t* words, uint64_t nwords, uint64_t offset)
+{
+  int dims = n[0] ? n[1] ? n[2] ? n[3] ? 4 : 3 : 2 : 1 : 0;
+  Params z{minbits, maxbits, maxprec, minexp};
+  switch (type) {
+    case 1: return dec_t<i32>(dims, data, n, s, z, words, nwords, offset);
+    case 2: return dec_t<i64>(dims, data, n, s, z, words, nwords, offset);
+    case 3: return dec_t<float>(dims, data, n, s, z, words, nwords, offset);
+    default: return dec_t<double>(dims, data, n, s, z, words, nwords, offset);
+  }
+}

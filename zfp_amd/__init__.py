@@ -1,0 +1,213 @@
+"""zfp_amd -- zfp's block codec on AMD Instinct MI355X (gfx950).
+
+The product is two in-tree shared libraries built from zfp_amd/csrc:
+
+* libzfp_hip.so  the gfx950 codec behind the C ABI of include/zfp_hip.h
+  (zfp_hip_compress / zfp_hip_decompress take zfp_stream/zfp_field like the
+  reference's cuda_compress / cuda_decompress; zfp_hip_encode / zfp_hip_decode
+  take device pointers and plain sizes);
+* libzfp_amd.so  zfp's high-level C API (include/zfp.h, same names, structs
+  and return values as LLNL zfp 1.0.1 src/zfp.c) with the zfp_exec_hip policy.
+
+This module loads them with ctypes and declares their signatures; it is a
+binding, not a second implementation.  Loading fails loudly when the
+libraries are missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_HIP = os.path.join(HERE, "libzfp_hip.so")
+LIB_API = os.path.join(HERE, "libzfp_amd.so")
+
+# enums of include/zfp.h
+zfp_exec_serial, zfp_exec_omp, zfp_exec_cuda, zfp_exec_hip = 0, 1, 2, 3
+zfp_type_none, zfp_type_int32, zfp_type_int64, zfp_type_float, zfp_type_double = 0, 1, 2, 3, 4
+(zfp_mode_null, zfp_mode_expert, zfp_mode_fixed_rate, zfp_mode_fixed_precision,
+ zfp_mode_fixed_accuracy, zfp_mode_reversible) = range(6)
+ZFP_HEADER_NONE, ZFP_HEADER_MAGIC, ZFP_HEADER_META, ZFP_HEADER_MODE, ZFP_HEADER_FULL = 0, 1, 2, 4, 7
+ZFP_MIN_BITS, ZFP_MAX_BITS, ZFP_MAX_PREC, ZFP_MIN_EXP = 1, 16658, 64, -1074
+
+ZFP_HIP_OK, ZFP_HIP_EINVAL, ZFP_HIP_ENODEV, ZFP_HIP_ENOMEM, ZFP_HIP_ERUNTIME, ZFP_HIP_ENOINDEX = range(6)
+
+
+class zfp_execution(ctypes.Structure):
+    _fields_ = [("policy", ctypes.c_int), ("params", ctypes.c_void_p)]
+
+
+class zfp_stream(ctypes.Structure):
+    """include/zfp.h:86-93 of the reference."""
+    _fields_ = [("minbits", ctypes.c_uint), ("maxbits", ctypes.c_uint), ("maxprec", ctypes.c_uint),
+                ("minexp", ctypes.c_int), ("stream", ctypes.c_void_p), ("exec", zfp_execution)]
+
+
+class zfp_field(ctypes.Structure):
+    """include/zfp.h:131-136 of the reference."""
+    _fields_ = [("type", ctypes.c_int), ("nx", ctypes.c_size_t), ("ny", ctypes.c_size_t),
+                ("nz", ctypes.c_size_t), ("nw", ctypes.c_size_t), ("sx", ctypes.c_ssize_t),
+                ("sy", ctypes.c_ssize_t), ("sz", ctypes.c_ssize_t), ("sw", ctypes.c_ssize_t),
+                ("data", ctypes.c_void_p)]
+
+
+class bitstream(ctypes.Structure):
+    """State layout of include/zfp/bitstream.inl:133-143 of the reference."""
+    _fields_ = [("bits", ctypes.c_size_t), ("buffer", ctypes.c_uint64), ("ptr", ctypes.c_void_p),
+                ("begin", ctypes.c_void_p), ("end", ctypes.c_void_p)]
+
+
+_P = ctypes.POINTER
+zs_p = _P(zfp_stream)
+zf_p = _P(zfp_field)
+bs_p = _P(bitstream)
+
+# name: (restype, argtypes)
+_API = {
+    "zfp_type_size": (ctypes.c_size_t, [ctypes.c_int]),
+    "zfp_stream_open": (zs_p, [bs_p]),
+    "zfp_stream_close": (None, [zs_p]),
+    "zfp_stream_bit_stream": (bs_p, [zs_p]),
+    "zfp_stream_compression_mode": (ctypes.c_int, [zs_p]),
+    "zfp_stream_rate": (ctypes.c_double, [zs_p, ctypes.c_uint]),
+    "zfp_stream_precision": (ctypes.c_uint, [zs_p]),
+    "zfp_stream_accuracy": (ctypes.c_double, [zs_p]),
+    "zfp_stream_mode": (ctypes.c_uint64, [zs_p]),
+    "zfp_stream_params": (None, [zs_p, _P(ctypes.c_uint), _P(ctypes.c_uint), _P(ctypes.c_uint), _P(ctypes.c_int)]),
+    "zfp_stream_compressed_size": (ctypes.c_size_t, [zs_p]),
+    "zfp_stream_maximum_size": (ctypes.c_size_t, [zs_p, zf_p]),
+    "zfp_stream_rewind": (None, [zs_p]),
+    "zfp_stream_set_bit_stream": (None, [zs_p, bs_p]),
+    "zfp_stream_set_reversible": (None, [zs_p]),
+    "zfp_stream_set_rate": (ctypes.c_double, [zs_p, ctypes.c_double, ctypes.c_int, ctypes.c_uint, ctypes.c_int]),
+    "zfp_stream_set_precision": (ctypes.c_uint, [zs_p, ctypes.c_uint]),
+    "zfp_stream_set_accuracy": (ctypes.c_double, [zs_p, ctypes.c_double]),
+    "zfp_stream_set_mode": (ctypes.c_int, [zs_p, ctypes.c_uint64]),
+    "zfp_stream_set_params": (ctypes.c_int, [zs_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_int]),
+    "zfp_stream_execution": (ctypes.c_int, [zs_p]),
+    "zfp_stream_set_execution": (ctypes.c_int, [zs_p, ctypes.c_int]),
+    "zfp_stream_set_omp_threads": (ctypes.c_int, [zs_p, ctypes.c_uint]),
+    "zfp_stream_omp_threads": (ctypes.c_uint, [zs_p]),
+    "zfp_field_alloc": (zf_p, []),
+    "zfp_field_1d": (zf_p, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]),
+    "zfp_field_2d": (zf_p, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t]),
+    "zfp_field_3d": (zf_p, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t]),
+    "zfp_field_4d": (zf_p, [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                            ctypes.c_size_t]),
+    "zfp_field_free": (None, [zf_p]),
+    "zfp_field_begin": (ctypes.c_void_p, [zf_p]),
+    "zfp_field_size_bytes": (ctypes.c_size_t, [zf_p]),
+    "zfp_field_blocks": (ctypes.c_size_t, [zf_p]),
+    "zfp_field_dimensionality": (ctypes.c_uint, [zf_p]),
+    "zfp_field_metadata": (ctypes.c_uint64, [zf_p]),
+    "zfp_field_set_metadata": (ctypes.c_int, [zf_p, ctypes.c_uint64]),
+    "zfp_field_set_pointer": (None, [zf_p, ctypes.c_void_p]),
+    "zfp_field_set_stride_1d": (None, [zf_p, ctypes.c_ssize_t]),
+    "zfp_field_set_stride_2d": (None, [zf_p, ctypes.c_ssize_t, ctypes.c_ssize_t]),
+    "zfp_field_set_stride_3d": (None, [zf_p, ctypes.c_ssize_t, ctypes.c_ssize_t, ctypes.c_ssize_t]),
+    "zfp_field_set_stride_4d": (None, [zf_p, ctypes.c_ssize_t, ctypes.c_ssize_t, ctypes.c_ssize_t,
+                                       ctypes.c_ssize_t]),
+    "zfp_compress": (ctypes.c_size_t, [zs_p, zf_p]),
+    "zfp_decompress": (ctypes.c_size_t, [zs_p, zf_p]),
+    "zfp_write_header": (ctypes.c_size_t, [zs_p, zf_p, ctypes.c_uint]),
+    "zfp_read_header": (ctypes.c_size_t, [zs_p, zf_p, ctypes.c_uint]),
+    "zfp_stream_flush": (ctypes.c_size_t, [zs_p]),
+    "zfp_stream_align": (ctypes.c_size_t, [zs_p]),
+    "stream_open": (bs_p, [ctypes.c_void_p, ctypes.c_size_t]),
+    "stream_close": (None, [bs_p]),
+    "stream_size": (ctypes.c_size_t, [bs_p]),
+    "stream_wtell": (ctypes.c_uint64, [bs_p]),
+    "stream_rtell": (ctypes.c_uint64, [bs_p]),
+    "stream_rewind": (None, [bs_p]),
+    "stream_write_bits": (ctypes.c_uint64, [bs_p, ctypes.c_uint64, ctypes.c_size_t]),
+    "stream_read_bits": (ctypes.c_uint64, [bs_p, ctypes.c_size_t]),
+    "stream_write_bit": (ctypes.c_uint, [bs_p, ctypes.c_uint]),
+    "stream_read_bit": (ctypes.c_uint, [bs_p]),
+    "stream_pad": (None, [bs_p, ctypes.c_uint64]),
+    "stream_skip": (None, [bs_p, ctypes.c_uint64]),
+    "stream_flush": (ctypes.c_size_t, [bs_p]),
+    "stream_align": (ctypes.c_size_t, [bs_p]),
+    "stream_rseek": (None, [bs_p, ctypes.c_uint64]),
+    "stream_wseek": (None, [bs_p, ctypes.c_uint64]),
+    "stream_copy": (None, [bs_p, bs_p, ctypes.c_uint64]),
+}
+
+_SZ4 = ctypes.c_size_t * 4
+_PD4 = ctypes.c_ssize_t * 4
+
+_HIP = {
+    "zfp_hip_compress": (ctypes.c_size_t, [zs_p, zf_p]),
+    "zfp_hip_decompress": (ctypes.c_size_t, [zs_p, zf_p]),
+    "zfp_hip_encode": (ctypes.c_int, [ctypes.c_int, _SZ4, _PD4, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
+                                      ctypes.c_uint, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                                      _P(ctypes.c_uint64), ctypes.c_void_p]),
+    "zfp_hip_decode": (ctypes.c_int, [ctypes.c_int, _SZ4, _PD4, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
+                                      ctypes.c_uint, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                                      _P(ctypes.c_uint64), ctypes.c_void_p]),
+    "zfp_hip_index_size": (ctypes.c_size_t, [ctypes.c_void_p]),
+    "zfp_hip_index_export": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), ctypes.c_size_t]),
+    "zfp_hip_index_import": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), ctypes.c_size_t]),
+    "zfp_hip_index_clear": (None, []),
+    "zfp_hip_device_check": (ctypes.c_int, []),
+    "zfp_hip_version": (ctypes.c_char_p, []),
+}
+
+_libs = {}
+
+
+def _load(path, table):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{os.path.basename(path)} is not built (run __graft_entry__.build() "
+                           f"or `make -C zfp_amd`); the HIP codec has no fallback")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in table.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def hip_lib():
+    """libzfp_hip.so: the codec's C ABI (include/zfp_hip.h)."""
+    if "hip" not in _libs:
+        _libs["hip"] = _load(LIB_HIP, _HIP)
+    return _libs["hip"]
+
+
+def api_lib():
+    """libzfp_amd.so: zfp's high-level C API with zfp_exec_hip (include/zfp.h)."""
+    if "api" not in _libs:
+        hip_lib()
+        _libs["api"] = _load(LIB_API, _API)
+    return _libs["api"]
+
+
+def exported_symbols():
+    """The C symbols each library must export (for load checks)."""
+    return {"libzfp_hip.so": sorted(_HIP), "libzfp_amd.so": sorted(_API)}
+
+
+def _args4(n, s):
+    nl = list(n) + [0] * (4 - len(n))
+    sl = list(s or []) + [0] * (4 - len(s or []))
+    return _SZ4(*nl), _PD4(*sl)
+
+
+def encode(ztype, n, s, d_data, params, d_words, offset=0, stream=None):
+    """Device-level encode (zfp_hip_encode); returns bits written."""
+    nn, ss = _args4(n, s)
+    bits = ctypes.c_uint64()
+    rc = hip_lib().zfp_hip_encode(ztype, nn, ss, ctypes.c_void_p(d_data), *params, ctypes.c_void_p(d_words),
+                                  offset, ctypes.byref(bits), ctypes.c_void_p(stream or 0))
+    if rc:
+        raise RuntimeError(f"zfp_hip_encode failed with status {rc}")
+    return bits.value
+
+
+def decode(ztype, n, s, d_data, params, d_words, offset=0, stream=None):
+    """Device-level decode (zfp_hip_decode); returns bits consumed."""
+    nn, ss = _args4(n, s)
+    bits = ctypes.c_uint64()
+    rc = hip_lib().zfp_hip_decode(ztype, nn, ss, ctypes.c_void_p(d_data), *params, ctypes.c_void_p(d_words),
+                                  offset, ctypes.byref(bits), ctypes.c_void_p(stream or 0))
+    if rc:
+        raise RuntimeError(f"zfp_hip_decode failed with status {rc}")
+    return bits.value

@@ -1,0 +1,72 @@
+// backend.h -- host-side declarations shared by the launchers
+// (codec_kernels.h, instantiated per dimensionality) and the C ABI
+// (zfp_hip.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "codec_block.h"
+#include "zfp_hip.h"
+
+namespace zfphip {
+
+// a field: type, dimensionality, geometry and block count
+struct Desc {
+  int type;
+  int dims;
+  Geom g;
+  u32 nblocks;
+  size_t esize;
+};
+
+#define HIPCHK(x) do { if ((x) != hipSuccess) { (void)hipGetLastError(); return ZFP_HIP_ERUNTIME; } } while (0)
+
+// grow-only device buffer
+struct Scratch {
+  void* p = nullptr;
+  size_t n = 0;
+  int get(size_t bytes, void** out)
+  {
+    if (bytes > n) {
+      if (p)
+        (void)hipFree(p);
+      p = nullptr;
+      n = 0;
+      if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return ZFP_HIP_ENOMEM;
+      }
+      n = bytes;
+    }
+    *out = p;
+    return ZFP_HIP_OK;
+  }
+};
+
+// scratch of the variable-rate passes (lengths + offsets, scan tiles); all
+// entry points run under one library lock, so one set suffices
+extern Scratch g_lens, g_scan_tmp;
+
+// launch the codec for one field; defined in codec_kernels.h and instantiated
+// for D = 1..4 in inst_d<D>.hip.  On variable-rate encodes, *boffs (when
+// non-null) receives a device copy of the nblocks+1 block offsets that the
+// caller must hipFree.
+template <int D>
+int dispatch_encode_d(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
+                      u64* bits, hipStream_t st, u64** boffs);
+template <int D>
+int dispatch_decode_d(const Desc& d, void* data, const Params& z, const u64* in, u64 nwords,
+                      u64 offset, const u64* boffs, u64* bits, hipStream_t st);
+
+extern template int dispatch_encode_d<1>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**);
+extern template int dispatch_encode_d<2>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**);
+extern template int dispatch_encode_d<3>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**);
+extern template int dispatch_encode_d<4>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**);
+extern template int dispatch_decode_d<1>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
+extern template int dispatch_decode_d<2>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
+extern template int dispatch_decode_d<3>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
+extern template int dispatch_decode_d<4>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
+
+} // namespace zfphip

@@ -1,0 +1,818 @@
+// codec_device.h -- device-side building blocks of the zfp block codec for
+// gfx950: gather with partial-block padding, block-floating-point cast in the
+// integer domain, lifting transform, negabinary reorder, bit-plane transpose
+// and the embedded plane coder, plus bit I/O on 64-bit stream words.
+//
+// Layout: one lane owns one zfp block.  A block's 4^d values live in VGPRs
+// (fully unrolled loops, compile-time indices), so the transform, the reorder
+// and the 32x32 bit-matrix transposes that turn coefficients into bit planes
+// are register renames plus ALU work, never memory traffic.
+//
+// Every routine cites the reference code whose result it reproduces
+// (LLNL zfp 1.0.1, paths relative to its tree).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zfphip {
+
+typedef uint64_t u64;
+typedef int64_t i64;
+typedef uint32_t u32;
+typedef int32_t i32;
+
+#define ZH_INLINE __device__ __forceinline__
+
+// --------------------------------------------------------------------------
+// scalar traits (src/traitsf.h, traitsd.h, traitsi.h, traitsl.h)
+// --------------------------------------------------------------------------
+template <typename S> struct Traits;
+template <> struct Traits<float> {
+  typedef i32 Int; typedef u32 UInt;
+  static constexpr int P = 32, EBITS = 8, EBIAS = 127, PBITS = 5;
+  static constexpr bool FLOAT = true;
+};
+template <> struct Traits<double> {
+  typedef i64 Int; typedef u64 UInt;
+  static constexpr int P = 64, EBITS = 11, EBIAS = 1023, PBITS = 6;
+  static constexpr bool FLOAT = true;
+};
+template <> struct Traits<i32> {
+  typedef i32 Int; typedef u32 UInt;
+  static constexpr int P = 32, EBITS = 0, EBIAS = 0, PBITS = 5;
+  static constexpr bool FLOAT = false;
+};
+template <> struct Traits<i64> {
+  typedef i64 Int; typedef u64 UInt;
+  static constexpr int P = 64, EBITS = 0, EBIAS = 0, PBITS = 6;
+  static constexpr bool FLOAT = false;
+};
+
+template <typename UInt> struct NB;
+template <> struct NB<u32> { static constexpr u32 mask = 0xaaaaaaaau; };
+template <> struct NB<u64> { static constexpr u64 mask = 0xaaaaaaaaaaaaaaaaull; };
+
+// --------------------------------------------------------------------------
+// coefficient orderings: format constants of the stream
+// (src/template/codec1.c:2-4, codec2.c:4-30, codec3.c:4-88, codec4.c:4-295)
+// --------------------------------------------------------------------------
+struct Order {
+  static constexpr unsigned char o2[16] = {0, 1, 4, 5, 2, 8, 6, 9, 3, 12, 10, 7, 13, 11, 14, 15};
+  static constexpr unsigned char o3[64] = {
+    0, 1, 4, 16, 20, 17, 5, 2, 8, 32, 21, 6, 18, 24, 9, 33,
+    36, 3, 12, 48, 22, 25, 37, 40, 34, 10, 7, 19, 28, 13, 49, 52,
+    41, 38, 26, 23, 29, 53, 11, 35, 44, 14, 50, 56, 42, 27, 39, 45,
+    30, 54, 57, 60, 51, 15, 43, 46, 58, 61, 55, 31, 62, 59, 47, 63};
+  static constexpr unsigned char o4[256] = {
+    0, 1, 4, 16, 64, 5, 80, 17, 68, 65, 20, 2, 8, 32, 128, 84,
+    81, 69, 21, 6, 18, 66, 24, 72, 9, 96, 33, 36, 129, 132, 144, 3,
+    12, 48, 192, 85, 82, 70, 22, 73, 25, 88, 37, 100, 97, 148, 145, 133,
+    10, 160, 34, 136, 130, 40, 7, 19, 67, 28, 76, 13, 112, 49, 52, 193,
+    196, 208, 86, 89, 101, 149, 161, 137, 41, 134, 38, 164, 26, 152, 146, 104,
+    98, 74, 83, 71, 23, 77, 29, 92, 53, 116, 113, 212, 209, 197, 11, 35,
+    131, 44, 140, 14, 176, 50, 56, 194, 200, 224, 90, 165, 102, 153, 150, 105,
+    168, 162, 138, 42, 87, 93, 117, 213, 27, 75, 99, 39, 135, 147, 108, 45,
+    141, 156, 30, 78, 177, 180, 54, 114, 120, 57, 198, 210, 216, 201, 225, 228,
+    15, 240, 51, 204, 195, 60, 169, 166, 154, 106, 91, 103, 151, 109, 157, 94,
+    181, 118, 121, 214, 217, 229, 163, 139, 43, 142, 46, 172, 58, 184, 178, 232,
+    226, 202, 241, 205, 61, 199, 55, 244, 31, 220, 211, 124, 115, 79, 170, 167,
+    155, 107, 158, 110, 173, 122, 185, 182, 233, 230, 218, 95, 245, 119, 221, 215,
+    125, 242, 206, 62, 203, 59, 248, 47, 236, 227, 188, 179, 143, 171, 174, 186,
+    234, 246, 222, 126, 219, 123, 249, 111, 237, 231, 189, 183, 159, 252, 243, 207,
+    63, 175, 250, 187, 238, 235, 190, 253, 247, 223, 127, 254, 251, 239, 191, 255};
+};
+
+template <int D> ZH_INLINE constexpr int order_at(int i)
+{
+  return D == 1 ? i : D == 2 ? Order::o2[i] : D == 3 ? Order::o3[i] : Order::o4[i];
+}
+
+// --------------------------------------------------------------------------
+// small helpers
+// --------------------------------------------------------------------------
+ZH_INLINE u64 lowmask(u32 n) { return n >= 64 ? ~0ull : ((1ull << n) - 1); }
+ZH_INLINE u64 shr64(u64 x, u32 n) { return n >= 64 ? 0ull : x >> n; }
+ZH_INLINE u64 shl64(u64 x, u32 n) { return n >= 64 ? 0ull : x << n; }
+ZH_INLINE u32 ctz64(u64 x) { return (u32)__builtin_ctzll(x); }
+ZH_INLINE u32 umin(u32 a, u32 b) { return a < b ? a : b; }
+
+// padded source position along one axis of a partial block: the reference
+// pads x, then y, z, w with p1 <- p0, p2 <- p1, p3 <- p0 patterns
+// (src/template/encode.c:8-27), which composes to this per-axis map
+ZH_INLINE int pad_pos(int p, int m)
+{
+  return m >= 4 ? p : m == 3 ? (p == 3 ? 0 : p) : m == 2 ? (p == 2 ? 1 : (p == 3 ? 0 : p)) : 0;
+}
+
+// --------------------------------------------------------------------------
+// block geometry
+// --------------------------------------------------------------------------
+struct Geom {
+  i64 s[4];   // strides in elements
+  u32 n[4];   // extents (1 for unused dims)
+  u32 nb[4];  // blocks per dim
+  u32 dims;
+};
+
+// Gather the 4^D values of block (bx,by,bz,bw) into v[] with partial-block
+// padding (encode1.c:4-20, encode2.c:4-25, encode3.c:4-31, encode4.c:4-38).
+template <typename S, int D>
+ZH_INLINE void gather(S* v, const S* data, const Geom& g, const u32* b)
+{
+  constexpr int N = 1 << (2 * D);
+  int m[4];
+  bool full = true;
+  const S* p = data;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    u32 o = 4 * b[d];
+    m[d] = (int)umin(4u, g.n[d] - o);
+    full = full && m[d] == 4;
+    p += (i64)o * g.s[d];
+  }
+  if (full) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      i64 off = (i64)(i & 3) * g.s[0];
+      if (D > 1) off += (i64)((i >> 2) & 3) * g.s[1];
+      if (D > 2) off += (i64)((i >> 4) & 3) * g.s[2];
+      if (D > 3) off += (i64)((i >> 6) & 3) * g.s[3];
+      v[i] = p[off];
+    }
+  }
+  else {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      i64 off = (i64)pad_pos(i & 3, m[0]) * g.s[0];
+      if (D > 1) off += (i64)pad_pos((i >> 2) & 3, m[1]) * g.s[1];
+      if (D > 2) off += (i64)pad_pos((i >> 4) & 3, m[2]) * g.s[2];
+      if (D > 3) off += (i64)pad_pos((i >> 6) & 3, m[3]) * g.s[3];
+      v[i] = p[off];
+    }
+  }
+}
+
+// Write back only the valid part of a block (decode1.c:4-19 ... decode4.c:4-25).
+template <typename S, int D>
+ZH_INLINE void scatter(const S* v, S* data, const Geom& g, const u32* b)
+{
+  constexpr int N = 1 << (2 * D);
+  u32 m[4] = {4, 4, 4, 4};
+  S* p = data;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    u32 o = 4 * b[d];
+    m[d] = umin(4u, g.n[d] - o);
+    p += (i64)o * g.s[d];
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    u32 x = i & 3, y = (i >> 2) & 3, z = (i >> 4) & 3, w = (i >> 6) & 3;
+    bool ok = x < m[0];
+    i64 off = (i64)x * g.s[0];
+    if (D > 1) { ok = ok && y < m[1]; off += (i64)y * g.s[1]; }
+    if (D > 2) { ok = ok && z < m[2]; off += (i64)z * g.s[2]; }
+    if (D > 3) { ok = ok && w < m[3]; off += (i64)w * g.s[3]; }
+    if (ok)
+      p[off] = v[i];
+  }
+}
+
+// --------------------------------------------------------------------------
+// block-floating-point cast, computed on the IEEE bit patterns
+// --------------------------------------------------------------------------
+
+// frexp-style exponent of the largest |x| over the block, NaNs ignored,
+// 0 -> -EBIAS, subnormal -> 1-EBIAS, inf -> 0 (glibc frexp) -- the result of
+// src/template/encodef.c:10-40.  IEEE magnitudes order like their bit
+// patterns, so the max is an integer max of the sign-stripped bits.
+template <int N>
+ZH_INLINE int block_emax(const double* v)
+{
+  u64 mx = 0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    u64 a = (u64)__double_as_longlong(v[i]) & 0x7fffffffffffffffull;
+    a = a > 0x7ff0000000000000ull ? 0ull : a; // NaN never wins
+    mx = a > mx ? a : mx;
+  }
+  if (!mx)
+    return -1023;
+  int ef = (int)(mx >> 52);
+  if (ef == 2047)
+    return 0;
+  return ef == 0 ? -1022 : ef - 1022;
+}
+
+template <int N>
+ZH_INLINE int block_emax(const float* v)
+{
+  u32 mx = 0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    u32 a = (u32)__float_as_uint(v[i]) & 0x7fffffffu;
+    a = a > 0x7f800000u ? 0u : a;
+    mx = a > mx ? a : mx;
+  }
+  if (!mx)
+    return -127;
+  int ef = (int)(mx >> 23);
+  if (ef == 255)
+    return 0;
+  return ef == 0 ? -126 : ef - 126;
+}
+
+// (Int)(2^(P-2-emax) * x) with truncation, computed exactly on the bits.
+// NaN, infinities, out-of-range products and an infinite scale factor
+// (emax < -961 for double, < -97 for float) all give INT_MIN, the value the
+// reference's conversion produces on x86-64 (src/template/encodef.c:42-59).
+ZH_INLINE i64 fwd_cast_one(double x, int emax)
+{
+  u64 bits = (u64)__double_as_longlong(x);
+  int ef = (int)((bits >> 52) & 0x7ff);
+  u64 man = bits & 0x000fffffffffffffull;
+  if (ef == 2047 || emax < -961)
+    return (i64)0x8000000000000000ull;
+  u64 m = ef ? (man | (1ull << 52)) : man;
+  int sh = (ef ? ef : 1) - 1013 - emax;
+  u64 mag;
+  if (sh >= 0) {
+    // m << sh must stay below 2^63
+    if (m && (64 - __builtin_clzll(m)) + sh > 63)
+      return (i64)0x8000000000000000ull;
+    mag = m << sh;
+  }
+  else
+    mag = shr64(m, (u32)(-sh));
+  return (bits >> 63) ? -(i64)mag : (i64)mag;
+}
+
+ZH_INLINE i32 fwd_cast_one(float x, int emax)
+{
+  u32 bits = __float_as_uint(x);
+  int ef = (int)((bits >> 23) & 0xff);
+  u32 man = bits & 0x007fffffu;
+  if (ef == 255 || emax < -97)
+    return (i32)0x80000000u;
+  u32 m = ef ? (man | (1u << 23)) : man;
+  int sh = (ef ? ef : 1) - 120 - emax;
+  u32 mag;
+  if (sh >= 0) {
+    if (m && (32 - __builtin_clz(m)) + sh > 31)
+      return (i32)0x80000000u;
+    mag = m << sh;
+  }
+  else
+    mag = (-sh) >= 32 ? 0u : (m >> (-sh));
+  return (bits >> 31) ? -(i32)mag : (i32)mag;
+}
+
+// 2^e as a double, exactly as ldexp(1.0, e) rounds it (0 below 2^-1074)
+ZH_INLINE double exp2_exact(int e)
+{
+  if (e >= -1022)
+    return __longlong_as_double((long long)((u64)(e + 1023) << 52));
+  if (e >= -1074)
+    return __longlong_as_double((long long)(1ull << (e + 1074)));
+  return 0.0;
+}
+
+ZH_INLINE float exp2_exact_f(int e)
+{
+  if (e >= -126)
+    return __uint_as_float((u32)(e + 127) << 23);
+  if (e >= -149)
+    return __uint_as_float(1u << (e + 149));
+  return 0.0f;
+}
+
+// (Scalar)(2^(emax-(P-2)) * (Scalar)i): round-to-nearest-even conversion,
+// then one IEEE multiply with subnormals preserved (src/template/codecf.c:15-32)
+ZH_INLINE double inv_cast_one(i64 i, double s) { return s * (double)i; }
+ZH_INLINE float inv_cast_one(i32 i, float s) { return s * (float)i; }
+
+// --------------------------------------------------------------------------
+// decorrelating transform (src/template/encode.c:30-56, decode.c:7-45)
+// wrap-around arithmetic in the coefficient width, arithmetic right shifts
+// --------------------------------------------------------------------------
+template <typename Int>
+ZH_INLINE void fwd_lift(Int& x, Int& y, Int& z, Int& w)
+{
+  typedef typename Traits<Int>::UInt U;
+  x = (Int)((U)x + (U)w); x >>= 1; w = (Int)((U)w - (U)x);
+  z = (Int)((U)z + (U)y); z >>= 1; y = (Int)((U)y - (U)z);
+  x = (Int)((U)x + (U)z); x >>= 1; z = (Int)((U)z - (U)x);
+  w = (Int)((U)w + (U)y); w >>= 1; y = (Int)((U)y - (U)w);
+  w = (Int)((U)w + (U)(y >> 1)); y = (Int)((U)y - (U)(w >> 1));
+}
+
+template <typename Int>
+ZH_INLINE void inv_lift(Int& x, Int& y, Int& z, Int& w)
+{
+  typedef typename Traits<Int>::UInt U;
+  y = (Int)((U)y + (U)(w >> 1)); w = (Int)((U)w - (U)(y >> 1));
+  y = (Int)((U)y + (U)w); w = (Int)((U)w - ((U)y - (U)w));
+  z = (Int)((U)z + (U)x); x = (Int)((U)x - ((U)z - (U)x));
+  y = (Int)((U)y + (U)z); z = (Int)((U)z - ((U)y - (U)z));
+  w = (Int)((U)w + (U)x); x = (Int)((U)x - ((U)w - (U)x));
+}
+
+// forward transform: axis x, then y, z, w (encode3.c:147-164 and kin)
+template <typename Int, int D>
+ZH_INLINE void fwd_xform(Int* q)
+{
+  constexpr int N = 1 << (2 * D);
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const int st = 1 << (2 * a);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      if (((i >> (2 * a)) & 3) == 0)
+        fwd_lift(q[i], q[i + st], q[i + 2 * st], q[i + 3 * st]);
+  }
+}
+
+// inverse transform: axes in reverse order (decode3.c:25-42 and kin)
+template <typename Int, int D>
+ZH_INLINE void inv_xform(Int* q)
+{
+  constexpr int N = 1 << (2 * D);
+#pragma unroll
+  for (int a = D - 1; a >= 0; a--) {
+    const int st = 1 << (2 * a);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      if (((i >> (2 * a)) & 3) == 0)
+        inv_lift(q[i], q[i + st], q[i + 2 * st], q[i + 3 * st]);
+  }
+}
+
+// --------------------------------------------------------------------------
+// bit-plane transpose: 32x32 bit matrices, least significant bit = column 0.
+// After transpose32(a), bit i of a[j] is bit j of the original a[i].
+// --------------------------------------------------------------------------
+template <int J, u32 M>
+ZH_INLINE void tp_stage(u32* a)
+{
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    if (k & J)
+      continue;
+    u32 t = ((a[k] >> J) ^ a[k + J]) & M;
+    a[k + J] ^= t;
+    a[k] ^= t << J;
+  }
+}
+
+ZH_INLINE void transpose32(u32* a)
+{
+  tp_stage<16, 0x0000ffffu>(a);
+  tp_stage<8, 0x00ff00ffu>(a);
+  tp_stage<4, 0x0f0f0f0fu>(a);
+  tp_stage<2, 0x33333333u>(a);
+  tp_stage<1, 0x55555555u>(a);
+}
+
+// Planes of N <= 64 coefficients: X[k] bit i = bit k of u[i].
+template <int N>
+ZH_INLINE void to_planes(const u64* u, u64* X)
+{
+  // 64-bit coefficients: four (or two, or direct) 32x32 transposes
+  if (N == 4) {
+#pragma unroll
+    for (int k = 0; k < 64; k++) {
+      u64 x = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        x |= ((u[i] >> k) & 1ull) << i;
+      X[k] = x;
+    }
+    return;
+  }
+  u32 a[32], b[32];
+  constexpr int H = N >= 32 ? 32 : N;
+#pragma unroll
+  for (int half = 0; half < 2; half++) { // half 0: bits 0..31, half 1: bits 32..63
+#pragma unroll
+    for (int i = 0; i < 32; i++)
+      a[i] = i < H ? (u32)(u[i] >> (32 * half)) : 0u;
+    transpose32(a);
+    if (N == 64) {
+#pragma unroll
+      for (int i = 0; i < 32; i++)
+        b[i] = (u32)(u[32 + i] >> (32 * half));
+      transpose32(b);
+    }
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+      X[32 * half + k] = (u64)a[k] | (N == 64 ? ((u64)b[k] << 32) : 0ull);
+  }
+}
+
+template <int N>
+ZH_INLINE void to_planes(const u32* u, u64* X)
+{
+  if (N == 4) {
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      u64 x = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        x |= (u64)((u[i] >> k) & 1u) << i;
+      X[k] = x;
+    }
+    return;
+  }
+  u32 a[32], b[32];
+  constexpr int H = N >= 32 ? 32 : N;
+#pragma unroll
+  for (int i = 0; i < 32; i++)
+    a[i] = i < H ? u[i] : 0u;
+  transpose32(a);
+  if (N == 64) {
+#pragma unroll
+    for (int i = 0; i < 32; i++)
+      b[i] = u[32 + i];
+    transpose32(b);
+  }
+#pragma unroll
+  for (int k = 0; k < 32; k++)
+    X[k] = (u64)a[k] | (N == 64 ? ((u64)b[k] << 32) : 0ull);
+}
+
+// inverse of to_planes
+template <int N>
+ZH_INLINE void from_planes(const u64* X, u64* u)
+{
+  if (N == 4) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      u64 v = 0;
+#pragma unroll
+      for (int k = 0; k < 64; k++)
+        v |= ((X[k] >> i) & 1ull) << k;
+      u[i] = v;
+    }
+    return;
+  }
+  u32 a[32], b[32];
+  constexpr int H = N >= 32 ? 32 : N;
+#pragma unroll
+  for (int i = 0; i < N; i++)
+    u[i] = 0;
+#pragma unroll
+  for (int half = 0; half < 2; half++) {
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      a[k] = (u32)X[32 * half + k];
+      b[k] = (u32)(X[32 * half + k] >> 32);
+    }
+    transpose32(a);
+    if (N == 64)
+      transpose32(b);
+#pragma unroll
+    for (int i = 0; i < H; i++)
+      u[i] |= (u64)a[i] << (32 * half);
+    if (N == 64) {
+#pragma unroll
+      for (int i = 0; i < 32; i++)
+        u[32 + i] |= (u64)b[i] << (32 * half);
+    }
+  }
+}
+
+template <int N>
+ZH_INLINE void from_planes(const u64* X, u32* u)
+{
+  if (N == 4) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      u32 v = 0;
+#pragma unroll
+      for (int k = 0; k < 32; k++)
+        v |= (u32)((X[k] >> i) & 1ull) << k;
+      u[i] = v;
+    }
+    return;
+  }
+  u32 a[32], b[32];
+  constexpr int H = N >= 32 ? 32 : N;
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    a[k] = (u32)X[k];
+    b[k] = (u32)(X[k] >> 32);
+  }
+  transpose32(a);
+  if (N == 64)
+    transpose32(b);
+#pragma unroll
+  for (int i = 0; i < H; i++)
+    u[i] = a[i];
+  if (N == 64) {
+#pragma unroll
+    for (int i = 0; i < 32; i++)
+      u[32 + i] = b[i];
+  }
+}
+
+// --------------------------------------------------------------------------
+// bit output: one lane appends to its own bit range [lo, hi) of the stream.
+// Words wholly inside the range are stored; boundary words, shared with a
+// neighbouring block or with bits already in the stream, are OR-ed into
+// memory the host has zeroed beforehand.
+// --------------------------------------------------------------------------
+struct BitWriter {
+  u64* out;
+  u64 wi;    // index of the word being filled
+  u64 acc;   // pending bits of word wi
+  u32 fill;  // number of pending bits (< 64)
+  u64 lo, hi;
+
+  ZH_INLINE void begin(u64* o, u64 start, u64 end)
+  {
+    out = o; wi = start >> 6; fill = (u32)(start & 63); acc = 0; lo = start; hi = end;
+  }
+  ZH_INLINE void emit(u64 word)
+  {
+    u64 b0 = wi << 6;
+    if (b0 >= lo && b0 + 64 <= hi)
+      out[wi] = word;
+    else if (word)
+      atomicOr((unsigned long long*)(out + wi), (unsigned long long)word);
+    wi++;
+  }
+  // append the low n (<= 64) bits of v; v must have no bits at or above n
+  ZH_INLINE void put(u64 v, u32 n)
+  {
+    acc |= v << fill;
+    u32 nf = fill + n;
+    if (nf >= 64) {
+      emit(acc);
+      acc = fill ? (v >> (64 - fill)) : 0ull;
+      nf -= 64;
+    }
+    fill = nf;
+  }
+  ZH_INLINE void zeros(u64 n)
+  {
+    while (n >= 64) { put(0, 64); n -= 64; }
+    if (n)
+      put(0, (u32)n);
+  }
+  ZH_INLINE void finish()
+  {
+    if (fill)
+      emit(acc);
+  }
+};
+
+// Counts bits without writing (length pass of variable-rate modes).
+struct BitCounter {
+  u64 pos;
+  ZH_INLINE void put(u64, u32 n) { pos += n; }
+  ZH_INLINE void zeros(u64 n) { pos += n; }
+};
+
+// Sequential reader of one lane's bits, with a bound on the words it may load.
+struct BitReader {
+  const u64* in;
+  u64 nwords;  // readable words
+  u64 next;    // next word index to load
+  u64 w0, w1;  // window; unread bits start at bit `pos` of w0
+  u32 pos;
+
+  ZH_INLINE u64 load(u64 i) const { return i < nwords ? in[i] : 0ull; }
+  ZH_INLINE void begin(const u64* p, u64 n, u64 bit)
+  {
+    in = p; nwords = n;
+    u64 wi = bit >> 6;
+    w0 = load(wi); w1 = load(wi + 1); next = wi + 2; pos = (u32)(bit & 63);
+  }
+  // next n (<= 64) bits without consuming them
+  ZH_INLINE u64 peek(u32 n) const
+  {
+    u64 v = pos ? ((w0 >> pos) | (w1 << (64 - pos))) : w0;
+    return v & lowmask(n);
+  }
+  ZH_INLINE void skip(u32 n) // n <= 64
+  {
+    pos += n;
+    if (pos >= 64) {
+      w0 = w1; w1 = load(next++); pos -= 64;
+    }
+  }
+  ZH_INLINE u64 read(u32 n) { u64 v = peek(n); skip(n); return v; }
+  ZH_INLINE void skip_long(u64 n)
+  {
+    while (n >= 64) { skip(64); n -= 64; }
+    if (n) skip((u32)n);
+  }
+};
+
+// --------------------------------------------------------------------------
+// embedded coder for N <= 64 coefficients given as bit planes X[k]
+// (src/template/encode.c:90-256; budget semantics of encode_few_ints:
+// every emitted bit costs one unit and coding stops when the budget is spent;
+// when the worst case fits the budget (codec.c:1-6) there is no budget).
+// Returns the number of bits written.
+// --------------------------------------------------------------------------
+ZH_INLINE bool budget_binds(u32 maxbits, u32 maxprec, u32 size)
+{
+  return (maxprec + 1) * size - 1 > maxbits;
+}
+
+template <int N, int P, typename W>
+ZH_INLINE u32 encode_planes(W& w, const u64* X, u32 maxbits, u32 maxprec)
+{
+  const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
+  u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
+  const u32 budget0 = budget;
+  u32 n = 0;
+#pragma unroll
+  for (int k = P - 1; k >= 0; k--) {
+    if ((u32)k >= kmin && budget) {
+      u64 x = X[k];
+      u32 m = umin(n, budget);
+      w.put(x & lowmask(m), m);
+      budget -= m;
+      x = shr64(x, n);
+      while (budget && n < (u32)N) {
+        budget--;
+        if (!x) { w.put(0, 1); break; }
+        w.put(1, 1);
+        u32 p = ctz64(x);
+        u32 cnt = (n + p < (u32)N - 1) ? p + 1 : p;
+        if (cnt > budget) { w.put(0, budget); budget = 0; break; }
+        w.put(cnt == p + 1 ? (1ull << p) : 0ull, cnt);
+        budget -= cnt;
+        n += p + 1;
+        x = shr64(x, p + 1);
+      }
+    }
+  }
+  return budget0 - budget;
+}
+
+// decoder mirror (src/template/decode.c:78-278).  Returns bits consumed.
+template <int N, int P>
+ZH_INLINE u32 decode_planes(BitReader& r, u64* X, u32 maxbits, u32 maxprec)
+{
+  const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
+  u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
+  const u32 budget0 = budget;
+  u32 n = 0;
+#pragma unroll
+  for (int k = P - 1; k >= 0; k--) {
+    u64 x = 0;
+    if ((u32)k >= kmin && budget) {
+      u32 m = umin(n, budget);
+      x = r.read(m);
+      budget -= m;
+      while (budget && n < (u32)N) {
+        budget--;
+        if (!r.read(1))
+          break;
+        // scan for the next one-bit; running out of budget or reaching the
+        // last coefficient makes the current coefficient the one
+        u32 av = umin((u32)N - 1 - n, budget);
+        u64 look = r.peek(av);
+        if (look) {
+          u32 p = ctz64(look);
+          r.skip(p + 1);
+          budget -= p + 1;
+          n += p;
+        }
+        else {
+          r.skip(av);
+          budget -= av;
+          n += av;
+        }
+        x |= 1ull << n;
+        n++;
+      }
+    }
+    X[k] = x;
+  }
+  return budget0 - budget;
+}
+
+// --------------------------------------------------------------------------
+// 4D blocks (256 coefficients): planes are four 64-bit words; the coder
+// follows encode_many_ints / decode_many_ints (encode.c:134-176,
+// decode.c:133-184), which produce the same code as the N <= 64 variants.
+// The block stays in lane-private memory.
+// --------------------------------------------------------------------------
+template <typename UInt, int P>
+ZH_INLINE void plane256(const UInt* u, int k, u64* x)
+{
+  for (int c = 0; c < 4; c++) {
+    u64 v = 0;
+    for (int i = 0; i < 64; i++)
+      v |= (u64)((u[64 * c + i] >> k) & 1u) << i;
+    x[c] = v;
+  }
+}
+
+// bits of the 256-bit plane x starting at position n (n < 256), 64 at most
+ZH_INLINE u64 bits256(const u64* x, u32 n)
+{
+  u32 c = n >> 6, s = n & 63;
+  u64 v = x[c] >> s;
+  if (s && c < 3)
+    v |= x[c + 1] << (64 - s);
+  return v;
+}
+
+template <typename UInt, int P, typename W>
+ZH_INLINE u32 encode_planes256(W& w, const UInt* u, u32 maxbits, u32 maxprec)
+{
+  const u32 N = 256;
+  const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
+  u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
+  const u32 budget0 = budget;
+  u32 n = 0;
+  for (int k = P - 1; k >= (int)kmin && budget; k--) {
+    u64 x[4];
+    plane256<UInt, P>(u, k, x);
+    // verbatim part, 64 bits at a time
+    u32 m = umin(n, budget);
+    budget -= m;
+    for (u32 i = 0; i < m; i += 64) {
+      u32 c = umin(64u, m - i);
+      w.put(x[i >> 6] & lowmask(c), c);
+    }
+    while (budget && n < N) {
+      budget--;
+      // any one-bit at or above n?
+      bool any = false;
+      for (u32 c = n >> 6; c < 4; c++)
+        any = any || (c == (n >> 6) ? (x[c] >> (n & 63)) != 0 : x[c] != 0);
+      if (!any) { w.put(0, 1); break; }
+      w.put(1, 1);
+      // distance p to the next one-bit
+      u32 p = 0;
+      for (;;) {
+        u64 v = bits256(x, n + p);
+        if (v) { p += ctz64(v); break; }
+        p += 64 - ((n + p) & 63);
+      }
+      u32 cnt = (n + p < N - 1) ? p + 1 : p;
+      if (cnt > budget) { w.zeros(budget); budget = 0; break; }
+      if (cnt == p + 1) { w.zeros(p); w.put(1, 1); }
+      else w.zeros(p);
+      budget -= cnt;
+      n += p + 1;
+    }
+  }
+  return budget0 - budget;
+}
+
+template <typename UInt, int P>
+ZH_INLINE u32 decode_planes256(BitReader& r, UInt* u, u32 maxbits, u32 maxprec)
+{
+  const u32 N = 256;
+  const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
+  u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
+  const u32 budget0 = budget;
+  u32 n = 0;
+  for (u32 i = 0; i < N; i++)
+    u[i] = 0;
+  for (int k = P - 1; k >= (int)kmin && budget; k--) {
+    u32 m = umin(n, budget);
+    budget -= m;
+    for (u32 i = 0; i < m; i += 64) {
+      u32 c = umin(64u, m - i);
+      u64 v = r.read(c);
+      for (u32 j = 0; j < c; j++)
+        u[i + j] += (UInt)((v >> j) & 1u) << k;
+    }
+    while (budget && n < N) {
+      budget--;
+      if (!r.read(1))
+        break;
+      // scan, at most 64 bits per look
+      for (;;) {
+        u32 av = umin(umin(N - 1 - n, budget), 64u);
+        if (!av)
+          break;
+        u64 look = r.peek(av);
+        if (look) {
+          u32 p = ctz64(look);
+          r.skip(p + 1);
+          budget -= p + 1;
+          n += p;
+          break;
+        }
+        r.skip(av);
+        budget -= av;
+        n += av;
+      }
+      u[n] += (UInt)1 << k;
+      n++;
+    }
+  }
+  return budget0 - budget;
+}
+
+} // namespace zfphip

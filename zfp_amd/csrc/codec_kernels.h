@@ -1,0 +1,320 @@
+// codec_kernels.h -- kernel templates of the MI355X zfp block codec and the
+// host-side launchers that instantiate them.  Included by one translation unit
+// per dimensionality (inst_d1.hip .. inst_d4.hip) so that the 1D-4D kernel
+// families compile in parallel.
+//
+// Kernels (one lane = one zfp block, blocks in stream order):
+//   k_encode<S,D,MODE>  gather -> BFP cast -> lifting -> reorder -> planes ->
+//                       embedded coder, writing the block's bits at its offset
+//                       (fixed rate: off + b*maxbits; otherwise from a scan)
+//                       or, in the length pass, only counting them
+//   k_decode<S,D>       the inverse, reading each block from its offset
+//   k_parse<S,D>        one-lane sequential parse of block lengths for
+//                       variable-rate streams that arrive without an index
+//   k_prepare           zeroes the words that blocks will OR into
+//   k_scan_*            exclusive prefix sum of block lengths
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstring>
+
+#include "backend.h"
+#include "codec_block.h"
+#include "codec_device.h"
+
+namespace zfphip {
+
+enum { ENC_FIXED = 0, ENC_OFFSETS = 1, ENC_LENGTH = 2 };
+
+template <typename S, int D, int MODE>
+__global__ void __launch_bounds__(256)
+k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
+         u64* __restrict__ out, u64 offset, const u64* __restrict__ boffs, u64* __restrict__ lens)
+{
+  u32 b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks)
+    return;
+  u32 bc[4] = {0, 0, 0, 0};
+  block_coords<D>(b, g, bc);
+  if (MODE == ENC_LENGTH) {
+    BitCounter cnt;
+    cnt.pos = 0;
+    lens[b] = encode_block<S, D>(cnt, data, g, bc, z);
+    return;
+  }
+  u64 start, end;
+  if (MODE == ENC_FIXED) {
+    start = offset + (u64)b * z.maxbits;
+    end = start + z.maxbits;
+  }
+  else {
+    start = offset + boffs[b];
+    end = offset + boffs[b + 1];
+  }
+  BitWriter w;
+  w.begin(out, start, end);
+  encode_block<S, D>(w, data, g, bc, z);
+  w.finish();
+}
+
+template <typename S, int D>
+__global__ void __launch_bounds__(256)
+k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restrict__ in,
+         u64 nwords, u64 offset, const u64* __restrict__ boffs)
+{
+  u32 b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks)
+    return;
+  u32 bc[4] = {0, 0, 0, 0};
+  block_coords<D>(b, g, bc);
+  u64 start = offset + (boffs ? boffs[b] : (u64)b * z.maxbits);
+  BitReader r;
+  r.begin(in, nwords, start);
+  decode_block<S, D, true>(r, data, g, bc, z);
+}
+
+// One lane walks the stream and records every block's offset (relative to
+// `offset`); boffs receives nblocks + 1 entries.
+template <typename S, int D>
+__global__ void k_parse(Geom g, u32 nblocks, Params z, const u64* __restrict__ in, u64 nwords,
+                        u64 offset, u64* __restrict__ boffs)
+{
+  if (blockIdx.x || threadIdx.x)
+    return;
+  u64 pos = 0;
+  u32 bc[4] = {0, 0, 0, 0};
+  for (u32 b = 0; b < nblocks; b++) {
+    boffs[b] = pos;
+    BitReader r;
+    r.begin(in, nwords, offset + pos);
+    pos += decode_block<S, D, false>(r, (S*)nullptr, g, bc, z);
+  }
+  boffs[nblocks] = pos;
+}
+
+// zero words [w0, w1) of `out`, keeping the low `keep` bits of word w0
+static __global__ void k_prepare(u64* out, u64 w0, u64 w1, u32 keep)
+{
+  u64 i = w0 + (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w1)
+    return;
+  if (i == w0 && keep)
+    out[i] &= lowmask(keep);
+  else
+    out[i] = 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// exclusive prefix sum of u64 block lengths (three passes: per-tile scan,
+// serial scan of tile totals in one workgroup, add tile prefixes)
+// ---------------------------------------------------------------------------
+constexpr int SCAN_BS = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_BS * SCAN_ITEMS;
+
+__device__ __forceinline__ u64 wg_exclusive_scan(u64 v, u64* sh, u64* total)
+{
+  // inclusive Hillis-Steele scan over the workgroup in LDS
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int d = 1; d < SCAN_BS; d <<= 1) {
+    u64 a = t >= d ? sh[t - d] : 0ull;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  u64 inc = sh[t];
+  *total = sh[SCAN_BS - 1];
+  __syncthreads();
+  return inc - v;
+}
+
+static __global__ void __launch_bounds__(SCAN_BS) k_scan_tiles(const u64* in, u64* out, u64 n, u64* tile_sums)
+{
+  __shared__ u64 sh[SCAN_BS];
+  u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_ITEMS;
+  u64 v[SCAN_ITEMS];
+  u64 sum = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    v[i] = base + i < n ? in[base + i] : 0ull;
+    sum += v[i];
+  }
+  u64 total;
+  u64 run = wg_exclusive_scan(sum, sh, &total);
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    if (base + i < n)
+      out[base + i] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == 0)
+    tile_sums[blockIdx.x] = total;
+}
+
+static __global__ void __launch_bounds__(SCAN_BS) k_scan_sums(u64* sums, u64 ntiles)
+{
+  __shared__ u64 sh[SCAN_BS];
+  u64 carry = 0;
+  for (u64 c = 0; c < ntiles; c += SCAN_BS) {
+    u64 i = c + threadIdx.x;
+    u64 v = i < ntiles ? sums[i] : 0ull;
+    u64 total;
+    u64 ex = wg_exclusive_scan(v, sh, &total);
+    if (i < ntiles)
+      sums[i] = carry + ex;
+    carry += total;
+  }
+}
+
+static __global__ void __launch_bounds__(SCAN_BS) k_scan_add(u64* out, u64 n, const u64* tile_sums)
+{
+  u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_ITEMS;
+  u64 add = tile_sums[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++)
+    if (base + i < n)
+      out[base + i] += add;
+}
+
+static inline unsigned grid_for(u64 n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+static inline int exclusive_scan(const u64* in, u64* out, u64 n, hipStream_t st)
+{
+  u64 ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  void* tp;
+  int rc = g_scan_tmp.get(ntiles * sizeof(u64), &tp);
+  if (rc)
+    return rc;
+  u64* sums = (u64*)tp;
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_BS), 0, st, in, out, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BS), 0, st, sums, ntiles);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)ntiles), dim3(SCAN_BS), 0, st, out, n, (const u64*)sums);
+  HIPCHK(hipGetLastError());
+  return ZFP_HIP_OK;
+}
+
+template <typename S, int D>
+int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
+                         u64* bits_out, hipStream_t st, u64** boffs_out)
+{
+  const unsigned BS = 256;
+  const unsigned grid = grid_for(d.nblocks, BS);
+  const bool fixed = z.minbits == z.maxbits;
+  if (fixed) {
+    u64 total = (u64)d.nblocks * z.maxbits;
+    u64 w0 = offset >> 6, w1 = (offset + total + 63) >> 6;
+    if ((offset & 63) || (z.maxbits & 63))
+      hipLaunchKernelGGL(k_prepare, dim3(grid_for(w1 - w0, 256)), dim3(256), 0, st, out, w0, w1,
+                         (u32)(offset & 63));
+    hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+                       d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    *bits_out = total;
+    if (boffs_out)
+      *boffs_out = nullptr;
+    return ZFP_HIP_OK;
+  }
+  // variable rate: length pass, exclusive scan, prepare, write pass
+  u64 nb1 = (u64)d.nblocks + 1;
+  void* lp;
+  int rc = g_lens.get(2 * nb1 * sizeof(u64), &lp);
+  if (rc)
+    return rc;
+  u64* lens = (u64*)lp;
+  u64* offs = lens + nb1;
+  HIPCHK(hipMemsetAsync(lens + d.nblocks, 0, sizeof(u64), st));
+  hipLaunchKernelGGL((k_encode<S, D, ENC_LENGTH>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+                     d.nblocks, z, out, offset, (const u64*)nullptr, lens);
+  HIPCHK(hipGetLastError());
+  rc = exclusive_scan(lens, offs, nb1, st);
+  if (rc)
+    return rc;
+  u64 total = 0;
+  HIPCHK(hipMemcpyAsync(&total, offs + d.nblocks, sizeof(u64), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  u64 w0 = offset >> 6, w1 = (offset + total + 63) >> 6;
+  if (w1 > w0)
+    hipLaunchKernelGGL(k_prepare, dim3(grid_for(w1 - w0, 256)), dim3(256), 0, st, out, w0, w1,
+                       (u32)(offset & 63));
+  hipLaunchKernelGGL((k_encode<S, D, ENC_OFFSETS>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+                     d.nblocks, z, out, offset, (const u64*)offs, (u64*)nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  *bits_out = total;
+  if (boffs_out) {
+    // hand a private copy of the offsets to the caller (index registry)
+    u64* keep = nullptr;
+    if (hipMalloc(&keep, nb1 * sizeof(u64)) != hipSuccess) {
+      (void)hipGetLastError();
+      *boffs_out = nullptr;
+      return ZFP_HIP_OK;
+    }
+    HIPCHK(hipMemcpyAsync(keep, offs, nb1 * sizeof(u64), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *boffs_out = keep;
+  }
+  return ZFP_HIP_OK;
+}
+
+template <typename S, int D>
+int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64 nwords,
+                         u64 offset, const u64* boffs, u64* bits_out, hipStream_t st)
+{
+  const unsigned BS = 256;
+  const unsigned grid = grid_for(d.nblocks, BS);
+  const bool fixed = z.minbits == z.maxbits;
+  u64 total = 0;
+  if (!fixed && !boffs) {
+    // no index: recover block offsets by a sequential parse
+    void* lp;
+    int rc = g_lens.get(((u64)d.nblocks + 1) * sizeof(u64), &lp);
+    if (rc)
+      return rc;
+    hipLaunchKernelGGL((k_parse<S, D>), dim3(1), dim3(64), 0, st, d.g, d.nblocks, z, in, nwords, offset,
+                       (u64*)lp);
+    HIPCHK(hipGetLastError());
+    boffs = (const u64*)lp;
+  }
+  hipLaunchKernelGGL((k_decode<S, D>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z, in,
+                     nwords, offset, fixed ? (const u64*)nullptr : boffs);
+  HIPCHK(hipGetLastError());
+  if (fixed)
+    total = (u64)d.nblocks * z.maxbits;
+  else
+    HIPCHK(hipMemcpyAsync(&total, boffs + d.nblocks, sizeof(u64), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *bits_out = total;
+  return ZFP_HIP_OK;
+}
+
+template <int D>
+int dispatch_encode_d(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
+                             u64* bits, hipStream_t st, u64** boffs)
+{
+  switch (d.type) {
+    case zfp_type_int32: return launch_encode<i32, D>(d, data, z, out, offset, bits, st, boffs);
+    case zfp_type_int64: return launch_encode<i64, D>(d, data, z, out, offset, bits, st, boffs);
+    case zfp_type_float: return launch_encode<float, D>(d, data, z, out, offset, bits, st, boffs);
+    case zfp_type_double: return launch_encode<double, D>(d, data, z, out, offset, bits, st, boffs);
+  }
+  return ZFP_HIP_EINVAL;
+}
+
+
+template <int D>
+int dispatch_decode_d(const Desc& d, void* data, const Params& z, const u64* in, u64 nwords,
+                             u64 offset, const u64* boffs, u64* bits, hipStream_t st)
+{
+  switch (d.type) {
+    case zfp_type_int32: return launch_decode<i32, D>(d, data, z, in, nwords, offset, boffs, bits, st);
+    case zfp_type_int64: return launch_decode<i64, D>(d, data, z, in, nwords, offset, boffs, bits, st);
+    case zfp_type_float: return launch_decode<float, D>(d, data, z, in, nwords, offset, boffs, bits, st);
+    case zfp_type_double: return launch_decode<double, D>(d, data, z, in, nwords, offset, boffs, bits, st);
+  }
+  return ZFP_HIP_EINVAL;
+}
+
+
+} // namespace zfphip

@@ -1,0 +1,539 @@
+// zfp_hip.hip -- C ABI of the MI355X zfp block codec (include/zfp_hip.h):
+// argument checking, host/device memory handling, stream-state bookkeeping
+// and the per-stream block-index registry used to decode variable-rate
+// streams in parallel.  The kernels live in codec_kernels.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "backend.h"
+#include "zfp_hip.h"
+
+namespace zfphip {
+
+Scratch g_lens, g_scan_tmp;
+
+namespace {
+
+Scratch g_field, g_stream;
+std::mutex g_mutex;
+
+int make_desc(int type, const size_t* n, const ptrdiff_t* s, Desc& d)
+{
+  memset(&d, 0, sizeof(d));
+  d.type = type;
+  switch (type) {
+    case zfp_type_int32: case zfp_type_float: d.esize = 4; break;
+    case zfp_type_int64: case zfp_type_double: d.esize = 8; break;
+    default: return ZFP_HIP_EINVAL;
+  }
+  // dimensionality as zfp_field_dimensionality (src/zfp.c:211-214)
+  d.dims = n[0] ? n[1] ? n[2] ? n[3] ? 4 : 3 : 2 : 1 : 0;
+  if (!d.dims)
+    return ZFP_HIP_EINVAL;
+  // default strides (src/zfp.c:260-278)
+  i64 def[4];
+  def[0] = 1;
+  def[1] = (i64)n[0];
+  def[2] = (i64)(n[0] * (n[1] ? n[1] : 1));
+  def[3] = (i64)(n[0] * (n[1] ? n[1] : 1) * (n[2] ? n[2] : 1));
+  u64 nb = 1;
+  for (int i = 0; i < 4; i++) {
+    if (i < d.dims) {
+      if (n[i] > 0xffffffffull)
+        return ZFP_HIP_EINVAL;
+      d.g.n[i] = (u32)n[i];
+      d.g.s[i] = (s && s[i]) ? (i64)s[i] : def[i];
+      d.g.nb[i] = (u32)((n[i] + 3) / 4);
+      nb *= d.g.nb[i];
+    }
+    else {
+      d.g.n[i] = 1;
+      d.g.s[i] = 0;
+      d.g.nb[i] = 1;
+    }
+  }
+  if (nb > 0xffffffffull)
+    return ZFP_HIP_EINVAL;
+  d.g.dims = (u32)d.dims;
+  d.nblocks = (u32)nb;
+  return ZFP_HIP_OK;
+}
+
+int dispatch_encode(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits,
+                    hipStream_t st, u64** boffs)
+{
+  switch (d.dims) {
+    case 1: return dispatch_encode_d<1>(d, data, z, out, offset, bits, st, boffs);
+    case 2: return dispatch_encode_d<2>(d, data, z, out, offset, bits, st, boffs);
+    case 3: return dispatch_encode_d<3>(d, data, z, out, offset, bits, st, boffs);
+    case 4: return dispatch_encode_d<4>(d, data, z, out, offset, bits, st, boffs);
+  }
+  return ZFP_HIP_EINVAL;
+}
+
+int dispatch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64 nwords, u64 offset,
+                    const u64* boffs, u64* bits, hipStream_t st)
+{
+  switch (d.dims) {
+    case 1: return dispatch_decode_d<1>(d, data, z, in, nwords, offset, boffs, bits, st);
+    case 2: return dispatch_decode_d<2>(d, data, z, in, nwords, offset, boffs, bits, st);
+    case 3: return dispatch_decode_d<3>(d, data, z, in, nwords, offset, boffs, bits, st);
+    case 4: return dispatch_decode_d<4>(d, data, z, in, nwords, offset, boffs, bits, st);
+  }
+  return ZFP_HIP_EINVAL;
+}
+
+// Block index of a variable-rate stream: offsets[b] relative to `start`
+// (nblocks + 1 entries, on the device).  Keyed by the stream buffer's begin
+// pointer; an entry matches a decode only at the same start offset, block
+// count and codec parameters (type/dims/extents too unless imported).
+struct Index {
+  u64 start = 0;
+  u32 nblocks = 0;
+  int type = 0, dims = 0; // 0: imported, matches any field with nblocks blocks
+  size_t n[4] = {0, 0, 0, 0};
+  Params z{};
+  bool any_params = false;
+  u64* d_offsets = nullptr;
+};
+std::map<const void*, Index> g_index;
+
+void index_put(const void* key, const Index& ix)
+{
+  auto it = g_index.find(key);
+  if (it != g_index.end() && it->second.d_offsets)
+    (void)hipFree(it->second.d_offsets);
+  g_index[key] = ix;
+}
+
+const Index* index_find(const void* key, u64 start, const Desc& d, const size_t* n, const Params& z)
+{
+  auto it = g_index.find(key);
+  if (it == g_index.end())
+    return nullptr;
+  const Index& ix = it->second;
+  if (ix.start != start || ix.nblocks != d.nblocks)
+    return nullptr;
+  if (ix.type) {
+    if (ix.type != d.type || ix.dims != d.dims)
+      return nullptr;
+    for (int i = 0; i < 4; i++)
+      if (ix.n[i] != n[i])
+        return nullptr;
+  }
+  if (!ix.any_params && (ix.z.minbits != z.minbits || ix.z.maxbits != z.maxbits ||
+                         ix.z.maxprec != z.maxprec || ix.z.minexp != z.minexp))
+    return nullptr;
+  return &ix;
+}
+
+int g_dev_state = -1; // -1 unknown, 0 usable, else an error code
+
+int device_check()
+{
+  if (g_dev_state >= 0)
+    return g_dev_state;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+    (void)hipGetLastError();
+    return g_dev_state = ZFP_HIP_ENODEV;
+  }
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return g_dev_state = ZFP_HIP_ENODEV;
+  }
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return g_dev_state = ZFP_HIP_ENODEV;
+  return g_dev_state = ZFP_HIP_OK;
+}
+
+bool is_device_ptr(const void* p)
+{
+  if (!p)
+    return false;
+  hipPointerAttribute_t a;
+  memset(&a, 0, sizeof(a));
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// lowest and highest element offsets of a field from element 0 (src/zfp.c:18-39)
+void field_span(const Desc& d, i64* lo, i64* hi)
+{
+  i64 a = 0, b = 0;
+  for (int i = 0; i < d.dims; i++) {
+    i64 e = d.g.s[i] * (i64)(d.g.n[i] - 1);
+    if (e < 0)
+      a += e;
+    else
+      b += e;
+  }
+  *lo = a;
+  *hi = b;
+}
+
+// worst-case bits of one block (the per-block term of zfp_stream_maximum_size,
+// src/zfp.c:712-742)
+u64 max_block_bits(const Desc& d, const Params& z)
+{
+  u32 vals = 1u << (2 * d.dims);
+  u32 p = d.esize == 4 ? 32 : 64;
+  u32 hdr = d.type == zfp_type_float ? 9 : d.type == zfp_type_double ? 12 : 0;
+  u64 mb = hdr + vals - 1 + (u64)vals * (z.maxprec < p ? z.maxprec : p);
+  if (mb > z.maxbits)
+    mb = z.maxbits;
+  if (mb < z.minbits)
+    mb = z.minbits;
+  return mb;
+}
+
+// the reference's bit stream state (include/zfp/bitstream.inl:133-143)
+struct StreamState {
+  size_t bits;
+  u64 buffer;
+  u64* ptr;
+  u64* begin;
+  u64* end;
+};
+
+bool params_ok(const zfp_stream* zs)
+{
+  // reversible mode (minexp < ZFP_MIN_EXP) is declined by this policy
+  return zs->minexp >= ZFP_MIN_EXP && zs->minbits <= zs->maxbits && zs->maxprec && zs->maxprec <= 64;
+}
+
+} // namespace
+} // namespace zfphip
+
+using namespace zfphip;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" int zfp_hip_device_check(void)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return device_check();
+}
+
+extern "C" const char* zfp_hip_version(void)
+{
+  return "zfp-amd 0.1 (zfp codec 5, gfx950)";
+}
+
+extern "C" int zfp_hip_encode(int type, const size_t n[4], const ptrdiff_t s[4], const void* d_data,
+                              uint minbits, uint maxbits, uint maxprec, int minexp, uint64_t* d_words,
+                              uint64_t offset, uint64_t* bits_out, void* hip_stream)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  int rc = device_check();
+  if (rc)
+    return rc;
+  Desc d;
+  rc = make_desc(type, n, s, d);
+  if (rc)
+    return rc;
+  if (minexp < ZFP_MIN_EXP || minbits > maxbits || !maxprec || maxprec > 64 || !d_data || !d_words)
+    return ZFP_HIP_EINVAL;
+  Params z{minbits, maxbits, maxprec, minexp};
+  u64 bits = 0;
+  u64* boffs = nullptr;
+  rc = dispatch_encode(d, d_data, z, d_words, offset, &bits, (hipStream_t)hip_stream,
+                       minbits == maxbits ? nullptr : &boffs);
+  if (rc)
+    return rc;
+  if (boffs) {
+    Index ix;
+    ix.start = offset;
+    ix.nblocks = d.nblocks;
+    ix.type = type;
+    ix.dims = d.dims;
+    for (int i = 0; i < 4; i++)
+      ix.n[i] = n[i];
+    ix.z = z;
+    ix.d_offsets = boffs;
+    index_put(d_words, ix);
+  }
+  if (bits_out)
+    *bits_out = bits;
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4], void* d_data,
+                              uint minbits, uint maxbits, uint maxprec, int minexp,
+                              const uint64_t* d_words, uint64_t offset, uint64_t* bits_out,
+                              void* hip_stream)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  int rc = device_check();
+  if (rc)
+    return rc;
+  Desc d;
+  rc = make_desc(type, n, s, d);
+  if (rc)
+    return rc;
+  if (minexp < ZFP_MIN_EXP || minbits > maxbits || !maxprec || maxprec > 64 || !d_data || !d_words)
+    return ZFP_HIP_EINVAL;
+  Params z{minbits, maxbits, maxprec, minexp};
+  const u64* boffs = nullptr;
+  if (minbits != maxbits) {
+    const Index* ix = index_find(d_words, offset, d, n, z);
+    if (ix)
+      boffs = ix->d_offsets;
+  }
+  // the caller's buffer holds at least the worst-case stream; reads beyond
+  // it are never issued
+  u64 nwords = (offset + (u64)d.nblocks * max_block_bits(d, z) + 63) >> 6;
+  u64 bits = 0;
+  rc = dispatch_decode(d, d_data, z, d_words, nwords, offset, boffs, &bits, (hipStream_t)hip_stream);
+  if (rc)
+    return rc;
+  if (bits_out)
+    *bits_out = bits;
+  return ZFP_HIP_OK;
+}
+
+extern "C" size_t zfp_hip_compress(zfp_stream* zs, const zfp_field* field)
+{
+  if (!zs || !zs->stream || !field || !params_ok(zs))
+    return 0;
+  size_t n[4] = {field->nx, field->ny, field->nz, field->nw};
+  ptrdiff_t s[4] = {field->sx, field->sy, field->sz, field->sw};
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (device_check())
+    return 0;
+  Desc d;
+  if (make_desc(field->type, n, s, d) || !field->data)
+    return 0;
+  StreamState* bs = (StreamState*)zs->stream;
+  // write position (include/zfp/bitstream.inl:323-327): whole words already
+  // in memory plus `lead` bits still buffered in the stream state
+  const u64 word0 = (u64)(bs->ptr - bs->begin);
+  const u32 lead = (u32)bs->bits;
+  Params z{zs->minbits, zs->maxbits, zs->maxprec, zs->minexp};
+  hipStream_t st = nullptr;
+
+  // field on the device (host fields are staged by their memory span)
+  const void* d_data = field->data;
+  if (!is_device_ptr(field->data)) {
+    i64 lo, hi;
+    field_span(d, &lo, &hi);
+    size_t bytes = (size_t)(hi - lo + 1) * d.esize;
+    void* p;
+    if (g_field.get(bytes, &p))
+      return 0;
+    const char* hbegin = (const char*)field->data + lo * (i64)d.esize;
+    if (hipMemcpy(p, hbegin, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+    d_data = (const char*)p - lo * (i64)d.esize;
+  }
+
+  // output: words from the current one, whose low `lead` bits are the
+  // buffered ones
+  const bool dev_stream = is_device_ptr(bs->begin);
+  const u64 cap_words = ((u64)lead + (u64)d.nblocks * max_block_bits(d, z) + 63) / 64 + 1;
+  u64* d_out;
+  if (dev_stream) {
+    d_out = bs->ptr;
+  }
+  else {
+    void* p;
+    if (g_stream.get(cap_words * sizeof(u64), &p))
+      return 0;
+    d_out = (u64*)p;
+  }
+  if (hipMemcpy(d_out, &bs->buffer, sizeof(u64), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  u64 bits = 0;
+  u64* boffs = nullptr;
+  const bool fixed = z.minbits == z.maxbits;
+  if (dispatch_encode(d, d_data, z, d_out, lead, &bits, st, fixed ? nullptr : &boffs))
+    return 0;
+  const u64 words = (lead + bits + 63) / 64;
+  if (!dev_stream &&
+      hipMemcpy(bs->ptr, d_out, words * sizeof(u64), hipMemcpyDeviceToHost) != hipSuccess) {
+    (void)hipGetLastError();
+    if (boffs)
+      (void)hipFree(boffs);
+    return 0;
+  }
+  if (boffs) {
+    Index ix;
+    ix.start = word0 * 64 + lead;
+    ix.nblocks = d.nblocks;
+    ix.type = field->type;
+    ix.dims = d.dims;
+    for (int i = 0; i < 4; i++)
+      ix.n[i] = n[i];
+    ix.z = z;
+    ix.d_offsets = boffs;
+    index_put(bs->begin, ix);
+  }
+  // leave the stream as serial compression plus stream_flush leaves it
+  bs->ptr += words;
+  bs->bits = 0;
+  bs->buffer = 0;
+  return (size_t)(bs->ptr - bs->begin) * sizeof(u64);
+}
+
+extern "C" size_t zfp_hip_decompress(zfp_stream* zs, zfp_field* field)
+{
+  if (!zs || !zs->stream || !field || !params_ok(zs))
+    return 0;
+  size_t n[4] = {field->nx, field->ny, field->nz, field->nw};
+  ptrdiff_t s[4] = {field->sx, field->sy, field->sz, field->sw};
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (device_check())
+    return 0;
+  Desc d;
+  if (make_desc(field->type, n, s, d) || !field->data)
+    return 0;
+  StreamState* bs = (StreamState*)zs->stream;
+  // read position (include/zfp/bitstream.inl:316-320)
+  const u64 rpos = (u64)(bs->ptr - bs->begin) * 64 - bs->bits;
+  const u64 word0 = rpos >> 6;
+  const u32 lead = (u32)(rpos & 63);
+  Params z{zs->minbits, zs->maxbits, zs->maxprec, zs->minexp};
+  const bool fixed = z.minbits == z.maxbits;
+  hipStream_t st = nullptr;
+
+  const Index* ix = fixed ? nullptr : index_find(bs->begin, rpos, d, n, z);
+  const u64* boffs = ix ? ix->d_offsets : nullptr;
+
+  // readable words: up to the end of the stream buffer, or of the stream
+  u64 avail = bs->end > bs->begin + word0 ? (u64)(bs->end - bs->begin) - word0 : 0;
+  u64 need = std::min<u64>(avail, (lead + (u64)d.nblocks * max_block_bits(d, z) + 63) / 64);
+  const u64* d_in;
+  if (is_device_ptr(bs->begin)) {
+    d_in = bs->begin + word0;
+  }
+  else {
+    void* p;
+    if (g_stream.get((need ? need : 1) * sizeof(u64), &p))
+      return 0;
+    if (need && hipMemcpy(p, bs->begin + word0, need * sizeof(u64), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+    d_in = (const u64*)p;
+  }
+
+  void* d_data = field->data;
+  char* hbegin = nullptr;
+  size_t span_bytes = 0;
+  void* dbuf = nullptr;
+  if (!is_device_ptr(field->data)) {
+    i64 lo, hi;
+    field_span(d, &lo, &hi);
+    span_bytes = (size_t)(hi - lo + 1) * d.esize;
+    if (g_field.get(span_bytes, &dbuf))
+      return 0;
+    hbegin = (char*)field->data + lo * (i64)d.esize;
+    // a strided host layout with gaps is copied in first so the gaps survive
+    // the copy back
+    u64 values = (u64)d.g.n[0] * d.g.n[1] * d.g.n[2] * d.g.n[3];
+    if (values * d.esize != span_bytes &&
+        hipMemcpy(dbuf, hbegin, span_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+    d_data = (char*)dbuf - lo * (i64)d.esize;
+  }
+  u64 bits = 0;
+  if (dispatch_decode(d, d_data, z, d_in, need, lead, boffs, &bits, st))
+    return 0;
+  if (hbegin && hipMemcpy(hbegin, dbuf, span_bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  // leave the stream as serial decompression plus stream_align leaves it
+  const u64 end = rpos + bits;
+  bs->ptr = bs->begin + (end + 63) / 64;
+  bs->bits = 0;
+  bs->buffer = 0;
+  return (size_t)(bs->ptr - bs->begin) * sizeof(u64);
+}
+
+extern "C" size_t zfp_hip_index_size(const void* stream_begin)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  auto it = g_index.find(stream_begin);
+  return it == g_index.end() ? 0 : (size_t)it->second.nblocks + 1;
+}
+
+extern "C" int zfp_hip_index_export(const void* stream_begin, uint64_t* offsets, size_t count)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  auto it = g_index.find(stream_begin);
+  if (it == g_index.end())
+    return ZFP_HIP_ENOINDEX;
+  const Index& ix = it->second;
+  if (!offsets || count < (size_t)ix.nblocks + 1)
+    return ZFP_HIP_EINVAL;
+  std::vector<u64> tmp((size_t)ix.nblocks + 1);
+  if (hipMemcpy(tmp.data(), ix.d_offsets, tmp.size() * sizeof(u64), hipMemcpyDeviceToHost) != hipSuccess) {
+    (void)hipGetLastError();
+    return ZFP_HIP_ERUNTIME;
+  }
+  for (size_t i = 0; i < tmp.size(); i++)
+    offsets[i] = ix.start + tmp[i];
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_index_import(const void* stream_begin, const uint64_t* offsets, size_t count)
+{
+  if (!stream_begin || !offsets || count < 2 || count - 1 > 0xffffffffull)
+    return ZFP_HIP_EINVAL;
+  for (size_t i = 1; i < count; i++)
+    if (offsets[i] < offsets[i - 1])
+      return ZFP_HIP_EINVAL;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  int rc = device_check();
+  if (rc)
+    return rc;
+  std::vector<u64> rel(count);
+  for (size_t i = 0; i < count; i++)
+    rel[i] = offsets[i] - offsets[0];
+  u64* dev = nullptr;
+  if (hipMalloc(&dev, count * sizeof(u64)) != hipSuccess) {
+    (void)hipGetLastError();
+    return ZFP_HIP_ENOMEM;
+  }
+  if (hipMemcpy(dev, rel.data(), count * sizeof(u64), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(dev);
+    return ZFP_HIP_ERUNTIME;
+  }
+  Index ix;
+  ix.start = offsets[0];
+  ix.nblocks = (u32)(count - 1);
+  ix.type = 0; // matches any field with this many blocks
+  ix.any_params = true;
+  ix.d_offsets = dev;
+  index_put(stream_begin, ix);
+  return ZFP_HIP_OK;
+}
+
+extern "C" void zfp_hip_index_clear(void)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  for (auto& kv : g_index)
+    if (kv.second.d_offsets)
+      (void)hipFree(kv.second.d_offsets);
+  g_index.clear();
+}
