@@ -1,0 +1,263 @@
+"""Benchmark of the MI355X zfp block codec.
+
+Workload (one GPU): fixed-rate compression and decompression of a 3D float64
+512^3 field (1 GiB) resident in HBM, at rates 4, 16 and 32 bits/value, for
+the contiguous layout and for the permuted view (z fastest in memory, x
+slowest with a negative stride).  The field is a seeded sum of 8 random 3D
+sinusoids (wavenumbers 1-6, amplitudes U(0.1, 1)) plus N(0, 1e-3) noise, with
+1% of the values replaced by 0, +-1e-300 and +-1e300.
+
+One step = compress + decompress at each of the three rates for both layouts
+(12 codec passes over 1 GiB).  `value` is the aggregate uncompressed
+throughput of those passes: 12 * 1 GiB / step time.  The roofline entry is
+for the dominant kernel (compression at rate 16, contiguous): algorithmic
+bytes N*8 + C per launch over its mean duration, timed with HIP events on
+the stream it is launched on.  The CPU baseline is the reference's serial
+zfp_compress/zfp_decompress (and its OpenMP compression) on a 128^3 slab of
+the same field, timed on this machine's host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--size", type=int, default=512)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline")
+    return p.parse_args()
+
+
+def make_field(torch, n, seed=20251202, special=0.01):
+    """Seeded smooth field on the device, stored x fastest ([z][y][x])."""
+    g = torch.Generator().manual_seed(seed)
+    k = torch.randint(1, 7, (8, 3), generator=g).double()
+    amp = 0.1 + 0.9 * torch.rand(8, generator=g, dtype=torch.float64)
+    ph = 2 * math.pi * torch.rand(8, generator=g, dtype=torch.float64)
+    dev = "cuda:0"
+    c = torch.arange(n, device=dev, dtype=torch.float64) * (2 * math.pi / n)
+    f = torch.zeros((n, n, n), device=dev, dtype=torch.float64)
+    for i in range(8):
+        ax = (k[i, 0] * c).view(1, 1, n)
+        ay = (k[i, 1] * c).view(1, n, 1)
+        az = (k[i, 2] * c).view(n, 1, 1)
+        f += amp[i].item() * torch.sin(ax + ay + az + ph[i].item())
+    gd = torch.Generator(device=dev).manual_seed(seed + 1)
+    f += 1e-3 * torch.randn((n, n, n), device=dev, dtype=torch.float64, generator=gd)
+    if special:
+        flat = f.view(-1)
+        pick = torch.rand(flat.numel(), device=dev, generator=gd) < special
+        choice = torch.randint(0, 5, (flat.numel(),), device=dev, generator=gd)
+        vals = torch.tensor([0.0, 1e-300, -1e-300, 1e300, -1e300], device=dev, dtype=torch.float64)
+        flat[pick] = vals[choice[pick]]
+    return f
+
+
+def gpu_bench(args):
+    import torch
+    import zfp_amd as Z
+
+    if Z.hip_lib().zfp_hip_device_check() != 0:
+        raise SystemExit("bench.py: no usable gfx950 device: " + Z.hip_lib().zfp_hip_last_error().decode())
+    n = args.size
+    N = n ** 3
+    f = make_field(torch, n)
+    # permuted view: element (x,y,z) at (n-1-x)*n*n + y*n + z
+    fp = torch.flip(f.permute(2, 1, 0), dims=[0]).contiguous()
+    torch.cuda.synchronize()
+    layouts = {
+        "contiguous": (f.data_ptr(), [0, 0, 0, 0]),
+        "permuted": (fp.data_ptr() + (n - 1) * n * n * 8, [-n * n, n, 1, 0]),
+    }
+    out = torch.empty_like(f)
+    rates = [4, 16, 32]
+    streams = {r: torch.zeros(N * r // 64 + 8, dtype=torch.int64, device="cuda:0") for r in rates}
+    st = torch.cuda.Stream()  # default priority
+    sh = st.cuda_stream
+    shape = [n, n, n]
+    params = {r: (64 * r, 64 * r, 64, -1074) for r in rates}  # zfp_stream_set_rate(r, double, 3D)
+
+    # decompression writes the field back in the layout it was read from
+    out_ptr = {"contiguous": out.data_ptr(), "permuted": out.data_ptr() + (n - 1) * n * n * 8}
+
+    def run_pass(op, r, layout):
+        ptr, s = layouts[layout]
+        if op == "compress":
+            return Z.encode(4, shape, s, ptr, params[r], streams[r].data_ptr(), 0, stream=sh)
+        return Z.decode(4, shape, s, out_ptr[layout], params[r], streams[r].data_ptr(), 0, stream=sh)
+
+    passes = [(op, r, lay) for lay in layouts for r in rates for op in ("compress", "decompress")]
+    # parity properties at full size: both layouts give the same stream, the
+    # fixed-rate size law holds, decompression consumes what compression
+    # wrote, and both layouts decompress to the same values
+    checks = {}
+    for r in rates:
+        bits_c = run_pass("compress", r, "contiguous")
+        s1 = streams[r].clone()
+        bits_p = run_pass("compress", r, "permuted")
+        same = bool(torch.equal(s1, streams[r]))
+        del s1
+        bits_d = run_pass("decompress", r, "contiguous")
+        o1 = out.clone()
+        run_pass("decompress", r, "permuted")
+        same_out = bool(torch.equal(o1.view(torch.int64), torch.flip(out.permute(2, 1, 0), dims=[2])
+                                    .contiguous().view(torch.int64)))
+        del o1
+        checks[f"rate{r}"] = {"bits": bits_c, "size_law": bits_c == N * r,
+                              "layouts_equal": same and bits_p == bits_c, "decode_bits": bits_d == bits_c,
+                              "layouts_decode_equal": same_out}
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        for p in passes:
+            run_pass(*p)
+    torch.cuda.synchronize()
+
+    # per-pass kernel timing with events on the launch stream
+    ev = {p: [] for p in passes}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for p in passes:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(st)
+            run_pass(*p)
+            b.record(st)
+            ev[p].append((a, b))
+    st.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    per = {}
+    for p, lst in ev.items():
+        per[p] = sum(a.elapsed_time(b) for a, b in lst) / len(lst)
+    ubytes = N * 8
+    value = len(passes) * ubytes / (ms_step * 1e-3) / 1e9
+    detail = {}
+    for (op, r, lay), ms in per.items():
+        cbytes = N * r // 8
+        detail[f"{op}_rate{r}_{lay}"] = {
+            "ms": round(ms, 4),
+            "GBps_uncompressed": round(ubytes / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round((ubytes + cbytes) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        }
+    dom = ("compress", 16, "contiguous")
+    dom_ms = per[dom]
+    traffic_bytes = ubytes + N * 16 // 8
+    achieved = traffic_bytes / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_encode<double,3,fixed> rate 16 contiguous",
+                "algorithmic_bytes_per_launch": traffic_bytes, "mean_launch_ms": round(dom_ms, 4)}
+    return value, ms_step, roofline, detail, checks, f
+
+
+def cpu_baseline(args, f_dev):
+    """Reference serial zfp on a 128^3 slab of the benchmark field (or the
+    oracle when the reference build is absent)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import zfptest as T
+    m = 128
+    slab = f_dev[:m, :m, :m].contiguous().cpu().numpy().reshape(-1)
+    n = [m, m, m]
+    rates = [4, 16, 32]
+    kind = "reference" if T.have_ref() else "port"
+    cores = os.cpu_count() or 1
+    res = {"serial_compress": [], "serial_decompress": [], "omp_compress": []}
+    t_budget = time.perf_counter() + args.cpu_seconds
+    reps = 0
+    while reps < 5 and (reps < 1 or time.perf_counter() < t_budget):
+        for r in rates:
+            p = T.stream_params_rate(r, T.T_DOUBLE, 3)
+            if kind == "reference":
+                api = T.ref_api()
+                t0 = time.perf_counter()
+                words, size = T.api_compress(api, slab, 0, n, None, T.T_DOUBLE, p)
+                t1 = time.perf_counter()
+                out = np.zeros_like(slab)
+                T.api_decompress(api, words, out, 0, n, None, T.T_DOUBLE, p)
+                t2 = time.perf_counter()
+                T.api_compress(api, slab, 0, n, None, T.T_DOUBLE, p, policy=1, threads=cores)
+                t3 = time.perf_counter()
+                res["omp_compress"].append((r, t3 - t2))
+            else:
+                t0 = time.perf_counter()
+                words, bits = T.oracle_compress(slab, 0, n, None, T.T_DOUBLE, p)
+                t1 = time.perf_counter()
+                out = np.zeros_like(slab)
+                T.oracle_decompress(words, out, 0, n, None, T.T_DOUBLE, p)
+                t2 = time.perf_counter()
+            res["serial_compress"].append((r, t1 - t0))
+            res["serial_decompress"].append((r, t2 - t1))
+        reps += 1
+    nbytes = slab.nbytes
+
+    def med(key):
+        ts = sorted(t for _, t in res[key])
+        return ts[len(ts) // 2] if ts else None
+
+    sc, sd = med("serial_compress"), med("serial_decompress")
+    # same definition as the GPU value: uncompressed bytes of compress+decompress passes / time
+    value = 2 * nbytes / (sc + sd) / 1e9
+    out = {"value": round(value, 4), "unit": "GB/s", "cores": 1, "kind": kind,
+           "sample": f"3D float64 128^3 slab of the workload field, rates 4/16/32, {reps} repetitions, "
+                     f"median of serial zfp_compress + zfp_decompress",
+           "serial_compress_GBps": round(nbytes / sc / 1e9, 4),
+           "serial_decompress_GBps": round(nbytes / sd / 1e9, 4)}
+    oc = med("omp_compress")
+    if oc:
+        out["omp_compress_GBps"] = round(nbytes / oc / 1e9, 4)
+        out["omp_threads"] = cores
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        raise SystemExit(2)
+    value, ms_step, roofline, detail, checks, f = gpu_bench(args)
+    cpu = None if args.no_cpu else cpu_baseline(args, f)
+    line = {
+        "metric": "compress/decompress GB/s (uncompressed) and % HBM peak; batched vs per-call",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"fixed-rate 3D float64 {args.size}^3 (1 GiB at 512^3), rates 4/16/32, "
+                               f"contiguous + permuted negative-stride view, compress + decompress",
+                   "passes_per_step": 12},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "passes": detail,
+        "parity_checks": checks,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

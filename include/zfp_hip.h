@@ -95,6 +95,10 @@ void zfp_hip_index_clear(void);
 int zfp_hip_device_check(void);
 const char* zfp_hip_version(void);
 
+/* Why the last call on this thread declined or failed ("" after a success);
+   the reference reports only a 0 return, this adds the reason. */
+const char* zfp_hip_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

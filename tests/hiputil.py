@@ -85,6 +85,7 @@ class Run:
             self.header_bits = api.zfp_write_header(self.zfp, self.field, self.header)
             assert self.header_bits > 0
         self.size = api.zfp_compress(self.zfp, self.field)
+        self.reason = Z.hip_lib().zfp_hip_last_error().decode()
         return self.size
 
     def stream_words(self):

@@ -1,0 +1,90 @@
+"""The codec's per-block device source (zfp_amd/csrc/codec_device.h and
+codec_block.h), compiled for the host by tests/hostcodec, against the oracle.
+
+This runs the very encode_block/decode_block templates the gfx950 kernels
+instantiate (gather with padding, integer-domain block-floating-point cast,
+lifting, reorder, 32x32 bit-plane transposes, plane coder, bit I/O), one
+block after another, so that logic errors show up without a GPU.  The GPU
+parity tests remain the proof for the device build itself."""
+import ctypes
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+from zfptest import (ROOT, T_DOUBLE, T_FLOAT, T_INT32, T_INT64, field_args, make_layout, oracle_compress,
+                     oracle_decompress, pdp, smooth_field, stream_params_accuracy, stream_params_precision,
+                     stream_params_rate, szp, u64p)
+
+LIB = os.path.join(ROOT, "tests", "hostcodec", "libhostcodec.so")
+pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="tests/hostcodec not built")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = ctypes.CDLL(LIB)
+        _lib.hostcodec_compress.restype = ctypes.c_uint64
+        _lib.hostcodec_compress.argtypes = [ctypes.c_int, szp, pdp, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
+                                            ctypes.c_uint, ctypes.c_int, u64p, ctypes.c_uint64]
+        _lib.hostcodec_decompress.restype = ctypes.c_uint64
+        _lib.hostcodec_decompress.argtypes = [ctypes.c_int, szp, pdp, ctypes.c_void_p, ctypes.c_uint,
+                                              ctypes.c_uint, ctypes.c_uint, ctypes.c_int, u64p, ctypes.c_uint64,
+                                              ctypes.c_uint64]
+    return _lib
+
+
+TYPES = {"i32": T_INT32, "i64": T_INT64, "f32": T_FLOAT, "f64": T_DOUBLE}
+SHAPES = {"1d": [103], "2d": [13, 11], "3d": [9, 10, 7], "4d": [5, 6, 7, 5], "3d_full": [16, 16, 8]}
+MODES = {
+    "rate8": lambda t, d: stream_params_rate(8, t, d),
+    "rate21": lambda t, d: stream_params_rate(21.5, t, d),
+    "rate5a": lambda t, d: stream_params_rate(5, t, d, True),
+    "prec16": lambda t, d: stream_params_precision(16),
+    "prec28": lambda t, d: stream_params_precision(28),
+    "acc1e-3": lambda t, d: stream_params_accuracy(1e-3),
+    "acc1e-7": lambda t, d: stream_params_accuracy(1e-7),
+    "expert": lambda t, d: (100, 600, 20, -30),
+}
+
+
+def compare(a, n, t, p, layout="as_is", offset=0):
+    base, off, s = make_layout(a, n, layout)
+    ew, eb = oracle_compress(base, off, n, s, t, p, offset=offset)
+    hw = np.zeros_like(ew)
+    nn, ss = field_args(n, s)
+    ptr = base.ctypes.data + off * base.itemsize
+    hb = lib().hostcodec_compress(t, nn, ss, ctypes.c_void_p(ptr), *p, hw.ctypes.data_as(u64p), offset)
+    assert hb == eb
+    assert np.array_equal(hw, ew)
+    ref = np.zeros_like(base)
+    oracle_decompress(ew, ref, off, n, s, t, p, offset=offset)
+    out = np.zeros_like(base)
+    db = lib().hostcodec_decompress(t, nn, ss, ctypes.c_void_p(out.ctypes.data + off * base.itemsize), *p,
+                                    ew.ctypes.data_as(u64p), ew.size, offset)
+    assert db == eb
+    assert np.array_equal(out.view(np.uint8), ref.view(np.uint8))
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+@pytest.mark.parametrize("tname", list(TYPES))
+@pytest.mark.parametrize("shape", list(SHAPES))
+def test_block_code_matches_oracle(shape, tname, mode):
+    n = SHAPES[shape]
+    t = TYPES[tname]
+    a = smooth_field(n, t, seed=zlib.crc32(f"{shape}/{tname}".encode()) & 0xffff)
+    compare(a, n, t, MODES[mode](t, len(n)))
+    if len(n) > 1:
+        compare(a, n, t, MODES[mode](t, len(n)), layout="permuted_neg", offset=77)
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+@pytest.mark.parametrize("tname", ["f32", "f64"])
+def test_special_values(tname, mode):
+    from test_oracle_vs_reference import _special
+    t = TYPES[tname]
+    a = _special(t)
+    compare(a, [4, 4, a.size // 16], t, MODES[mode](t, 3))

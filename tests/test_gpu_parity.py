@@ -56,7 +56,7 @@ def check(a, n, ztype, params, layout="as_is", header=0, device=True, device_str
     try:
         size = run.compress()
         exp_words, exp_size = oracle_stream(base, off, n, s, ztype, params, header)
-        assert size == exp_size, f"stream size {size} != serial {exp_size}"
+        assert size == exp_size, f"stream size {size} != serial {exp_size} ({run.reason})"
         got = run.stream_words()
         if not np.array_equal(got, exp_words):
             bad = np.nonzero(got != exp_words)[0]

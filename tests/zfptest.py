@@ -472,3 +472,85 @@ def smooth_field(n, ztype, seed, noise=1e-3, special=0.0):
     if ztype == T_INT32:
         return np.round(f * (1 << 26)).astype(np.int32)
     return np.round(f * (1 << 58)).astype(np.int64)
+
+
+# --------------------------------------------------------------------------
+# the reference libzfp (oracle/_ref/libzfp_ref.so) through its own C API
+# --------------------------------------------------------------------------
+
+_refapi = None
+
+
+def ref_api():
+    """The reference library, with the signatures of include/zfp.h (the same
+    table the product binding uses: both export the same names)."""
+    global _refapi
+    if _refapi is None:
+        import zfp_amd
+        lib = ctypes.CDLL(os.path.join(REF_DIR, "libzfp_ref.so"))
+        for name, (res, args) in zfp_amd._API.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _refapi = lib
+    return _refapi
+
+
+def api_compress(api, base, elem_offset, n, s, ztype, params, header=0, policy=0, threads=0):
+    """zfp_compress through a zfp C API (reference or ours) on host memory;
+    returns (stream words, size in bytes)."""
+    dims = len(n)
+    ptr = base.ctypes.data + elem_offset * base.itemsize
+    if dims == 1:
+        f = api.zfp_field_1d(ctypes.c_void_p(ptr), ztype, n[0])
+    elif dims == 2:
+        f = api.zfp_field_2d(ctypes.c_void_p(ptr), ztype, n[0], n[1])
+    elif dims == 3:
+        f = api.zfp_field_3d(ctypes.c_void_p(ptr), ztype, n[0], n[1], n[2])
+    else:
+        f = api.zfp_field_4d(ctypes.c_void_p(ptr), ztype, n[0], n[1], n[2], n[3])
+    if s and any(s):
+        ss = list(s) + [0] * (4 - len(s))
+        api.zfp_field_set_stride_4d(f, *ss)
+    z = api.zfp_stream_open(None)
+    assert api.zfp_stream_set_params(z, *params)
+    if policy == 1:
+        assert api.zfp_stream_set_omp_threads(z, threads)
+    cap = api.zfp_stream_maximum_size(z, f)
+    buf = np.zeros(cap // 8 + 1, dtype=np.uint64)
+    bs = api.stream_open(ctypes.c_void_p(buf.ctypes.data), cap)
+    api.zfp_stream_set_bit_stream(z, bs)
+    api.zfp_stream_rewind(z)
+    if header:
+        assert api.zfp_write_header(z, f, header)
+    size = api.zfp_compress(z, f)
+    api.stream_close(bs)
+    api.zfp_stream_close(z)
+    api.zfp_field_free(f)
+    return buf[: size // 8].copy(), size
+
+
+def api_decompress(api, words, base, elem_offset, n, s, ztype, params, header=0):
+    """zfp_decompress through a zfp C API on host memory; returns bytes consumed."""
+    dims = len(n)
+    ptr = base.ctypes.data + elem_offset * base.itemsize
+    nn = list(n) + [0] * (4 - dims)
+    f = api.zfp_field_4d(ctypes.c_void_p(ptr), ztype, *nn)
+    if s and any(s):
+        ss = list(s) + [0] * (4 - len(s))
+        api.zfp_field_set_stride_4d(f, *ss)
+    z = api.zfp_stream_open(None)
+    assert api.zfp_stream_set_params(z, *params)
+    w = np.ascontiguousarray(words, dtype=np.uint64).copy()
+    bs = api.stream_open(ctypes.c_void_p(w.ctypes.data), w.size * 8)
+    api.zfp_stream_set_bit_stream(z, bs)
+    api.zfp_stream_rewind(z)
+    if header:
+        g = api.zfp_field_alloc()
+        assert api.zfp_read_header(z, g, header)
+        api.zfp_field_free(g)
+    size = api.zfp_decompress(z, f)
+    api.stream_close(bs)
+    api.zfp_stream_close(z)
+    api.zfp_field_free(f)
+    return size

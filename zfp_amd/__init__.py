@@ -148,12 +148,27 @@ _HIP = {
     "zfp_hip_index_clear": (None, []),
     "zfp_hip_device_check": (ctypes.c_int, []),
     "zfp_hip_version": (ctypes.c_char_p, []),
+    "zfp_hip_last_error": (ctypes.c_char_p, []),
 }
 
 _libs = {}
 
 
+def _one_hip_runtime():
+    """A process must hold a single HIP runtime.  PyTorch-ROCm ships its own
+    libamdhip64 (SONAME libamdhip64.so.7, but linked by file name), so if our
+    library loaded /opt/rocm's copy first, a later `import torch` would map a
+    second runtime and one of the two would lose the device.  Importing
+    torch first (when it is installed) makes libzfp_hip.so bind to torch's
+    copy through the shared SONAME."""
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def _load(path, table):
+    _one_hip_runtime()
     if not os.path.exists(path):
         raise RuntimeError(f"{os.path.basename(path)} is not built (run __graft_entry__.build() "
                            f"or `make -C zfp_amd`); the HIP codec has no fallback")

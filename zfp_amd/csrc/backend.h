@@ -21,7 +21,18 @@ struct Desc {
   size_t esize;
 };
 
-#define HIPCHK(x) do { if ((x) != hipSuccess) { (void)hipGetLastError(); return ZFP_HIP_ERUNTIME; } } while (0)
+// Records a failed HIP call (message available from zfp_hip_last_error),
+// clears HIP's error state and returns ZFP_HIP_ERUNTIME.
+int record_hip_error(hipError_t e, const char* what);
+// Records a library-level failure reason; returns `code`.
+int record_error(int code, const char* what);
+
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return record_hip_error(e_, #x); } while (0)
+
+// zero words [w0, w1) of a stream, keeping the low `keep` bits of word w0
+__global__ void k_prepare(u64* out, u64 w0, u64 w1, u32 keep);
+// exclusive prefix sum of n u64 values (in and out may not alias)
+int exclusive_scan(const u64* in, u64* out, u64 n, hipStream_t st);
 
 // grow-only device buffer
 struct Scratch {
@@ -36,7 +47,8 @@ struct Scratch {
       n = 0;
       if (hipMalloc(&p, bytes) != hipSuccess) {
         (void)hipGetLastError();
-        return ZFP_HIP_ENOMEM;
+        p = nullptr;
+        return record_error(ZFP_HIP_ENOMEM, "device scratch allocation failed");
       }
       n = bytes;
     }

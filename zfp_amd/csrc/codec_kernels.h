@@ -92,107 +92,7 @@ __global__ void k_parse(Geom g, u32 nblocks, Params z, const u64* __restrict__ i
   boffs[nblocks] = pos;
 }
 
-// zero words [w0, w1) of `out`, keeping the low `keep` bits of word w0
-static __global__ void k_prepare(u64* out, u64 w0, u64 w1, u32 keep)
-{
-  u64 i = w0 + (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w1)
-    return;
-  if (i == w0 && keep)
-    out[i] &= lowmask(keep);
-  else
-    out[i] = 0;
-}
-
-
-// ---------------------------------------------------------------------------
-// exclusive prefix sum of u64 block lengths (three passes: per-tile scan,
-// serial scan of tile totals in one workgroup, add tile prefixes)
-// ---------------------------------------------------------------------------
-constexpr int SCAN_BS = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_BS * SCAN_ITEMS;
-
-__device__ __forceinline__ u64 wg_exclusive_scan(u64 v, u64* sh, u64* total)
-{
-  // inclusive Hillis-Steele scan over the workgroup in LDS
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int d = 1; d < SCAN_BS; d <<= 1) {
-    u64 a = t >= d ? sh[t - d] : 0ull;
-    __syncthreads();
-    sh[t] += a;
-    __syncthreads();
-  }
-  u64 inc = sh[t];
-  *total = sh[SCAN_BS - 1];
-  __syncthreads();
-  return inc - v;
-}
-
-static __global__ void __launch_bounds__(SCAN_BS) k_scan_tiles(const u64* in, u64* out, u64 n, u64* tile_sums)
-{
-  __shared__ u64 sh[SCAN_BS];
-  u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_ITEMS;
-  u64 v[SCAN_ITEMS];
-  u64 sum = 0;
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    v[i] = base + i < n ? in[base + i] : 0ull;
-    sum += v[i];
-  }
-  u64 total;
-  u64 run = wg_exclusive_scan(sum, sh, &total);
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    if (base + i < n)
-      out[base + i] = run;
-    run += v[i];
-  }
-  if (threadIdx.x == 0)
-    tile_sums[blockIdx.x] = total;
-}
-
-static __global__ void __launch_bounds__(SCAN_BS) k_scan_sums(u64* sums, u64 ntiles)
-{
-  __shared__ u64 sh[SCAN_BS];
-  u64 carry = 0;
-  for (u64 c = 0; c < ntiles; c += SCAN_BS) {
-    u64 i = c + threadIdx.x;
-    u64 v = i < ntiles ? sums[i] : 0ull;
-    u64 total;
-    u64 ex = wg_exclusive_scan(v, sh, &total);
-    if (i < ntiles)
-      sums[i] = carry + ex;
-    carry += total;
-  }
-}
-
-static __global__ void __launch_bounds__(SCAN_BS) k_scan_add(u64* out, u64 n, const u64* tile_sums)
-{
-  u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_ITEMS;
-  u64 add = tile_sums[blockIdx.x];
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++)
-    if (base + i < n)
-      out[base + i] += add;
-}
-
 static inline unsigned grid_for(u64 n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
-
-static inline int exclusive_scan(const u64* in, u64* out, u64 n, hipStream_t st)
-{
-  u64 ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-  void* tp;
-  int rc = g_scan_tmp.get(ntiles * sizeof(u64), &tp);
-  if (rc)
-    return rc;
-  u64* sums = (u64*)tp;
-  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_BS), 0, st, in, out, n, sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BS), 0, st, sums, ntiles);
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)ntiles), dim3(SCAN_BS), 0, st, out, n, (const u64*)sums);
-  HIPCHK(hipGetLastError());
-  return ZFP_HIP_OK;
-}
 
 template <typename S, int D>
 int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,

@@ -1,10 +1,11 @@
 // zfp_hip.hip -- C ABI of the MI355X zfp block codec (include/zfp_hip.h):
 // argument checking, host/device memory handling, stream-state bookkeeping
 // and the per-stream block-index registry used to decode variable-rate
-// streams in parallel.  The kernels live in codec_kernels.h.
+// streams in parallel.  The kernels live in codec_kernels.h and helpers.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -18,6 +19,23 @@ namespace zfphip {
 Scratch g_lens, g_scan_tmp;
 
 namespace {
+thread_local char g_last_error[256] = "";
+}
+
+int record_error(int code, const char* what)
+{
+  snprintf(g_last_error, sizeof(g_last_error), "%s", what);
+  return code;
+}
+
+int record_hip_error(hipError_t e, const char* what)
+{
+  (void)hipGetLastError();
+  snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
+  return ZFP_HIP_ERUNTIME;
+}
+
+namespace {
 
 Scratch g_field, g_stream;
 std::mutex g_mutex;
@@ -29,12 +47,12 @@ int make_desc(int type, const size_t* n, const ptrdiff_t* s, Desc& d)
   switch (type) {
     case zfp_type_int32: case zfp_type_float: d.esize = 4; break;
     case zfp_type_int64: case zfp_type_double: d.esize = 8; break;
-    default: return ZFP_HIP_EINVAL;
+    default: return record_error(ZFP_HIP_EINVAL, "unsupported scalar type");
   }
   // dimensionality as zfp_field_dimensionality (src/zfp.c:211-214)
   d.dims = n[0] ? n[1] ? n[2] ? n[3] ? 4 : 3 : 2 : 1 : 0;
   if (!d.dims)
-    return ZFP_HIP_EINVAL;
+    return record_error(ZFP_HIP_EINVAL, "field has no extent");
   // default strides (src/zfp.c:260-278)
   i64 def[4];
   def[0] = 1;
@@ -45,7 +63,7 @@ int make_desc(int type, const size_t* n, const ptrdiff_t* s, Desc& d)
   for (int i = 0; i < 4; i++) {
     if (i < d.dims) {
       if (n[i] > 0xffffffffull)
-        return ZFP_HIP_EINVAL;
+        return record_error(ZFP_HIP_EINVAL, "extent above 2^32-1");
       d.g.n[i] = (u32)n[i];
       d.g.s[i] = (s && s[i]) ? (i64)s[i] : def[i];
       d.g.nb[i] = (u32)((n[i] + 3) / 4);
@@ -58,7 +76,7 @@ int make_desc(int type, const size_t* n, const ptrdiff_t* s, Desc& d)
     }
   }
   if (nb > 0xffffffffull)
-    return ZFP_HIP_EINVAL;
+    return record_error(ZFP_HIP_EINVAL, "more than 2^32-1 blocks");
   d.g.dims = (u32)d.dims;
   d.nblocks = (u32)nb;
   return ZFP_HIP_OK;
@@ -95,7 +113,7 @@ int dispatch_decode(const Desc& d, void* data, const Params& z, const u64* in, u
 struct Index {
   u64 start = 0;
   u32 nblocks = 0;
-  int type = 0, dims = 0; // 0: imported, matches any field with nblocks blocks
+  int type = 0, dims = 0; // type 0: imported, matches any field with nblocks blocks
   size_t n[4] = {0, 0, 0, 0};
   Params z{};
   bool any_params = false;
@@ -134,23 +152,34 @@ const Index* index_find(const void* key, u64 start, const Desc& d, const size_t*
 
 int g_dev_state = -1; // -1 unknown, 0 usable, else an error code
 
+// called with the library lock held; also clears HIP error state left by
+// other users of the runtime in this thread, so that the checks after our
+// own calls see only our errors
 int device_check()
 {
-  if (g_dev_state >= 0)
+  (void)hipGetLastError();
+  if (g_dev_state >= 0) {
+    if (g_dev_state)
+      record_error(g_dev_state, "no usable gfx950 device");
     return g_dev_state;
+  }
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
     (void)hipGetLastError();
-    return g_dev_state = ZFP_HIP_ENODEV;
+    g_dev_state = ZFP_HIP_ENODEV;
+    return record_error(g_dev_state, "no HIP device");
   }
   int dev = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
     (void)hipGetLastError();
-    return g_dev_state = ZFP_HIP_ENODEV;
+    g_dev_state = ZFP_HIP_ENODEV;
+    return record_error(g_dev_state, "cannot query the HIP device");
   }
-  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-    return g_dev_state = ZFP_HIP_ENODEV;
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    g_dev_state = ZFP_HIP_ENODEV;
+    return record_error(g_dev_state, "the current HIP device is not gfx950");
+  }
   return g_dev_state = ZFP_HIP_OK;
 }
 
@@ -206,10 +235,25 @@ struct StreamState {
   u64* end;
 };
 
-bool params_ok(const zfp_stream* zs)
+const char* params_problem(const zfp_stream* zs)
 {
-  // reversible mode (minexp < ZFP_MIN_EXP) is declined by this policy
-  return zs->minexp >= ZFP_MIN_EXP && zs->minbits <= zs->maxbits && zs->maxprec && zs->maxprec <= 64;
+  if (zs->minexp < ZFP_MIN_EXP)
+    return "reversible mode is not offered by the HIP policy";
+  if (zs->minbits > zs->maxbits || !zs->maxprec || zs->maxprec > 64)
+    return "invalid compression parameters";
+  return nullptr;
+}
+
+size_t decline(const char* why)
+{
+  record_error(ZFP_HIP_EINVAL, why);
+  return 0;
+}
+
+size_t hip_failed(hipError_t e, const char* what)
+{
+  record_hip_error(e, what);
+  return 0;
 }
 
 } // namespace
@@ -232,6 +276,11 @@ extern "C" const char* zfp_hip_version(void)
   return "zfp-amd 0.1 (zfp codec 5, gfx950)";
 }
 
+extern "C" const char* zfp_hip_last_error(void)
+{
+  return g_last_error;
+}
+
 extern "C" int zfp_hip_encode(int type, const size_t n[4], const ptrdiff_t s[4], const void* d_data,
                               uint minbits, uint maxbits, uint maxprec, int minexp, uint64_t* d_words,
                               uint64_t offset, uint64_t* bits_out, void* hip_stream)
@@ -245,7 +294,7 @@ extern "C" int zfp_hip_encode(int type, const size_t n[4], const ptrdiff_t s[4],
   if (rc)
     return rc;
   if (minexp < ZFP_MIN_EXP || minbits > maxbits || !maxprec || maxprec > 64 || !d_data || !d_words)
-    return ZFP_HIP_EINVAL;
+    return record_error(ZFP_HIP_EINVAL, "invalid codec parameters or null pointer");
   Params z{minbits, maxbits, maxprec, minexp};
   u64 bits = 0;
   u64* boffs = nullptr;
@@ -284,7 +333,7 @@ extern "C" int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4],
   if (rc)
     return rc;
   if (minexp < ZFP_MIN_EXP || minbits > maxbits || !maxprec || maxprec > 64 || !d_data || !d_words)
-    return ZFP_HIP_EINVAL;
+    return record_error(ZFP_HIP_EINVAL, "invalid codec parameters or null pointer");
   Params z{minbits, maxbits, maxprec, minexp};
   const u64* boffs = nullptr;
   if (minbits != maxbits) {
@@ -306,16 +355,20 @@ extern "C" int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4],
 
 extern "C" size_t zfp_hip_compress(zfp_stream* zs, const zfp_field* field)
 {
-  if (!zs || !zs->stream || !field || !params_ok(zs))
-    return 0;
+  if (!zs || !zs->stream || !field)
+    return decline("null stream or field");
+  if (const char* why = params_problem(zs))
+    return decline(why);
   size_t n[4] = {field->nx, field->ny, field->nz, field->nw};
   ptrdiff_t s[4] = {field->sx, field->sy, field->sz, field->sw};
   std::lock_guard<std::mutex> lock(g_mutex);
   if (device_check())
     return 0;
   Desc d;
-  if (make_desc(field->type, n, s, d) || !field->data)
+  if (make_desc(field->type, n, s, d))
     return 0;
+  if (!field->data)
+    return decline("field has no data pointer");
   StreamState* bs = (StreamState*)zs->stream;
   // write position (include/zfp/bitstream.inl:323-327): whole words already
   // in memory plus `lead` bits still buffered in the stream state
@@ -323,6 +376,7 @@ extern "C" size_t zfp_hip_compress(zfp_stream* zs, const zfp_field* field)
   const u32 lead = (u32)bs->bits;
   Params z{zs->minbits, zs->maxbits, zs->maxprec, zs->minexp};
   hipStream_t st = nullptr;
+  hipError_t e;
 
   // field on the device (host fields are staged by their memory span)
   const void* d_data = field->data;
@@ -334,10 +388,8 @@ extern "C" size_t zfp_hip_compress(zfp_stream* zs, const zfp_field* field)
     if (g_field.get(bytes, &p))
       return 0;
     const char* hbegin = (const char*)field->data + lo * (i64)d.esize;
-    if (hipMemcpy(p, hbegin, bytes, hipMemcpyHostToDevice) != hipSuccess) {
-      (void)hipGetLastError();
-      return 0;
-    }
+    if ((e = hipMemcpy(p, hbegin, bytes, hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_failed(e, "copy of the field to the device");
     d_data = (const char*)p - lo * (i64)d.esize;
   }
 
@@ -355,22 +407,18 @@ extern "C" size_t zfp_hip_compress(zfp_stream* zs, const zfp_field* field)
       return 0;
     d_out = (u64*)p;
   }
-  if (hipMemcpy(d_out, &bs->buffer, sizeof(u64), hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipGetLastError();
-    return 0;
-  }
+  if ((e = hipMemcpy(d_out, &bs->buffer, sizeof(u64), hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_failed(e, "store of the buffered stream bits");
   u64 bits = 0;
   u64* boffs = nullptr;
   const bool fixed = z.minbits == z.maxbits;
   if (dispatch_encode(d, d_data, z, d_out, lead, &bits, st, fixed ? nullptr : &boffs))
     return 0;
   const u64 words = (lead + bits + 63) / 64;
-  if (!dev_stream &&
-      hipMemcpy(bs->ptr, d_out, words * sizeof(u64), hipMemcpyDeviceToHost) != hipSuccess) {
-    (void)hipGetLastError();
+  if (!dev_stream && (e = hipMemcpy(bs->ptr, d_out, words * sizeof(u64), hipMemcpyDeviceToHost)) != hipSuccess) {
     if (boffs)
       (void)hipFree(boffs);
-    return 0;
+    return hip_failed(e, "copy of the stream to the host");
   }
   if (boffs) {
     Index ix;
@@ -388,21 +436,26 @@ extern "C" size_t zfp_hip_compress(zfp_stream* zs, const zfp_field* field)
   bs->ptr += words;
   bs->bits = 0;
   bs->buffer = 0;
+  g_last_error[0] = 0;
   return (size_t)(bs->ptr - bs->begin) * sizeof(u64);
 }
 
 extern "C" size_t zfp_hip_decompress(zfp_stream* zs, zfp_field* field)
 {
-  if (!zs || !zs->stream || !field || !params_ok(zs))
-    return 0;
+  if (!zs || !zs->stream || !field)
+    return decline("null stream or field");
+  if (const char* why = params_problem(zs))
+    return decline(why);
   size_t n[4] = {field->nx, field->ny, field->nz, field->nw};
   ptrdiff_t s[4] = {field->sx, field->sy, field->sz, field->sw};
   std::lock_guard<std::mutex> lock(g_mutex);
   if (device_check())
     return 0;
   Desc d;
-  if (make_desc(field->type, n, s, d) || !field->data)
+  if (make_desc(field->type, n, s, d))
     return 0;
+  if (!field->data)
+    return decline("field has no data pointer");
   StreamState* bs = (StreamState*)zs->stream;
   // read position (include/zfp/bitstream.inl:316-320)
   const u64 rpos = (u64)(bs->ptr - bs->begin) * 64 - bs->bits;
@@ -411,6 +464,7 @@ extern "C" size_t zfp_hip_decompress(zfp_stream* zs, zfp_field* field)
   Params z{zs->minbits, zs->maxbits, zs->maxprec, zs->minexp};
   const bool fixed = z.minbits == z.maxbits;
   hipStream_t st = nullptr;
+  hipError_t e;
 
   const Index* ix = fixed ? nullptr : index_find(bs->begin, rpos, d, n, z);
   const u64* boffs = ix ? ix->d_offsets : nullptr;
@@ -426,10 +480,8 @@ extern "C" size_t zfp_hip_decompress(zfp_stream* zs, zfp_field* field)
     void* p;
     if (g_stream.get((need ? need : 1) * sizeof(u64), &p))
       return 0;
-    if (need && hipMemcpy(p, bs->begin + word0, need * sizeof(u64), hipMemcpyHostToDevice) != hipSuccess) {
-      (void)hipGetLastError();
-      return 0;
-    }
+    if (need && (e = hipMemcpy(p, bs->begin + word0, need * sizeof(u64), hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_failed(e, "copy of the stream to the device");
     d_in = (const u64*)p;
   }
 
@@ -448,24 +500,21 @@ extern "C" size_t zfp_hip_decompress(zfp_stream* zs, zfp_field* field)
     // the copy back
     u64 values = (u64)d.g.n[0] * d.g.n[1] * d.g.n[2] * d.g.n[3];
     if (values * d.esize != span_bytes &&
-        hipMemcpy(dbuf, hbegin, span_bytes, hipMemcpyHostToDevice) != hipSuccess) {
-      (void)hipGetLastError();
-      return 0;
-    }
+        (e = hipMemcpy(dbuf, hbegin, span_bytes, hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_failed(e, "copy of the field to the device");
     d_data = (char*)dbuf - lo * (i64)d.esize;
   }
   u64 bits = 0;
   if (dispatch_decode(d, d_data, z, d_in, need, lead, boffs, &bits, st))
     return 0;
-  if (hbegin && hipMemcpy(hbegin, dbuf, span_bytes, hipMemcpyDeviceToHost) != hipSuccess) {
-    (void)hipGetLastError();
-    return 0;
-  }
+  if (hbegin && (e = hipMemcpy(hbegin, dbuf, span_bytes, hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_failed(e, "copy of the field to the host");
   // leave the stream as serial decompression plus stream_align leaves it
   const u64 end = rpos + bits;
   bs->ptr = bs->begin + (end + 63) / 64;
   bs->bits = 0;
   bs->buffer = 0;
+  g_last_error[0] = 0;
   return (size_t)(bs->ptr - bs->begin) * sizeof(u64);
 }
 
@@ -481,15 +530,12 @@ extern "C" int zfp_hip_index_export(const void* stream_begin, uint64_t* offsets,
   std::lock_guard<std::mutex> lock(g_mutex);
   auto it = g_index.find(stream_begin);
   if (it == g_index.end())
-    return ZFP_HIP_ENOINDEX;
+    return record_error(ZFP_HIP_ENOINDEX, "no block index for this stream");
   const Index& ix = it->second;
   if (!offsets || count < (size_t)ix.nblocks + 1)
-    return ZFP_HIP_EINVAL;
+    return record_error(ZFP_HIP_EINVAL, "offset array too small");
   std::vector<u64> tmp((size_t)ix.nblocks + 1);
-  if (hipMemcpy(tmp.data(), ix.d_offsets, tmp.size() * sizeof(u64), hipMemcpyDeviceToHost) != hipSuccess) {
-    (void)hipGetLastError();
-    return ZFP_HIP_ERUNTIME;
-  }
+  HIPCHK(hipMemcpy(tmp.data(), ix.d_offsets, tmp.size() * sizeof(u64), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < tmp.size(); i++)
     offsets[i] = ix.start + tmp[i];
   return ZFP_HIP_OK;
@@ -498,10 +544,10 @@ extern "C" int zfp_hip_index_export(const void* stream_begin, uint64_t* offsets,
 extern "C" int zfp_hip_index_import(const void* stream_begin, const uint64_t* offsets, size_t count)
 {
   if (!stream_begin || !offsets || count < 2 || count - 1 > 0xffffffffull)
-    return ZFP_HIP_EINVAL;
+    return record_error(ZFP_HIP_EINVAL, "invalid index");
   for (size_t i = 1; i < count; i++)
     if (offsets[i] < offsets[i - 1])
-      return ZFP_HIP_EINVAL;
+      return record_error(ZFP_HIP_EINVAL, "index offsets decrease");
   std::lock_guard<std::mutex> lock(g_mutex);
   int rc = device_check();
   if (rc)
@@ -510,14 +556,11 @@ extern "C" int zfp_hip_index_import(const void* stream_begin, const uint64_t* of
   for (size_t i = 0; i < count; i++)
     rel[i] = offsets[i] - offsets[0];
   u64* dev = nullptr;
-  if (hipMalloc(&dev, count * sizeof(u64)) != hipSuccess) {
-    (void)hipGetLastError();
-    return ZFP_HIP_ENOMEM;
-  }
-  if (hipMemcpy(dev, rel.data(), count * sizeof(u64), hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipGetLastError();
+  HIPCHK(hipMalloc(&dev, count * sizeof(u64)));
+  hipError_t e = hipMemcpy(dev, rel.data(), count * sizeof(u64), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
     (void)hipFree(dev);
-    return ZFP_HIP_ERUNTIME;
+    return record_hip_error(e, "copy of the index to the device");
   }
   Index ix;
   ix.start = offsets[0];
