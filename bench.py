@@ -106,21 +106,22 @@ def gpu_bench(args):
     # fixed-rate size law holds, decompression consumes what compression
     # wrote, and both layouts decompress to the same values
     checks = {}
-    for r in rates:
-        bits_c = run_pass("compress", r, "contiguous")
-        s1 = streams[r].clone()
-        bits_p = run_pass("compress", r, "permuted")
-        same = bool(torch.equal(s1, streams[r]))
-        del s1
-        bits_d = run_pass("decompress", r, "contiguous")
-        o1 = out.clone()
-        run_pass("decompress", r, "permuted")
-        same_out = bool(torch.equal(o1.view(torch.int64), torch.flip(out.permute(2, 1, 0), dims=[2])
-                                    .contiguous().view(torch.int64)))
-        del o1
-        checks[f"rate{r}"] = {"bits": bits_c, "size_law": bits_c == N * r,
-                              "layouts_equal": same and bits_p == bits_c, "decode_bits": bits_d == bits_c,
-                              "layouts_decode_equal": same_out}
+    with torch.cuda.stream(st):  # torch's copies and the codec run in order on one stream
+        for r in rates:
+            bits_c = run_pass("compress", r, "contiguous")
+            s1 = streams[r].clone()
+            bits_p = run_pass("compress", r, "permuted")
+            same = bool(torch.equal(s1, streams[r]))
+            del s1
+            bits_d = run_pass("decompress", r, "contiguous")
+            o1 = out.clone()
+            run_pass("decompress", r, "permuted")
+            same_out = bool(torch.equal(o1.view(torch.int64), torch.flip(out.permute(2, 1, 0), dims=[2])
+                                        .contiguous().view(torch.int64)))
+            del o1
+            checks[f"rate{r}"] = {"bits": bits_c, "size_law": bits_c == N * r,
+                                  "layouts_equal": same and bits_p == bits_c, "decode_bits": bits_d == bits_c,
+                                  "layouts_decode_equal": same_out}
     torch.cuda.synchronize()
 
     for _ in range(args.warmup):

@@ -54,7 +54,7 @@ uint64_t enc(const void* data, const size_t* n, const ptrdiff_t* s, const Params
   for (u32 b = 0; b < nb; b++) {
     u32 bc[4] = {0, 0, 0, 0};
     block_coords<D>(b, g, bc);
-    BitWriter w;
+    BitWriter<false> w;
     w.begin(words, offset + off[b], offset + off[b + 1]);
     encode_block<S, D>(w, (const S*)data, g, bc, z);
     w.finish();

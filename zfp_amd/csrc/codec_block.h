@@ -39,7 +39,7 @@ __device__ __forceinline__ void block_coords(u32 b, const Geom& g, u32* c)
 // encode.c:259-280); returns the block's length in bits
 // ---------------------------------------------------------------------------
 template <typename S, int D, typename W>
-__device__ u32 encode_block(W& w, const S* data, const Geom& g, const u32* bc, const Params& z)
+__device__ __forceinline__ u32 encode_block(W& w, const S* data, const Geom& g, const u32* bc, const Params& z)
 {
   typedef Traits<S> T;
   typedef typename T::Int Int;
@@ -84,9 +84,29 @@ __device__ u32 encode_block(W& w, const S* data, const Geom& g, const u32* bc, c
     u[i] = ((UInt)q[order_at<D>(i)] + NB<UInt>::mask) ^ NB<UInt>::mask;
   u32 c;
   if constexpr (D < 4) {
+    PlaneEncoder<N> pe;
+    pe.init(T::P, maxbits, maxprec);
     u64 X[T::P];
-    to_planes<N>(u, X);
-    c = encode_planes<N, T::P>(w, X, maxbits, maxprec);
+    if constexpr (N == 64 && T::P == 64) {
+      // high planes first; the low-plane transposes only when still coding
+      planes64_half<1>(u, X);
+#pragma unroll
+      for (int k = 63; k >= 32; k--)
+        pe.plane(w, X[k], (u32)k);
+      if (pe.active() && pe.kmin < 32) {
+        planes64_half<0>(u, X);
+#pragma unroll
+        for (int k = 31; k >= 0; k--)
+          pe.plane(w, X[k], (u32)k);
+      }
+    }
+    else {
+      to_planes<N>(u, X);
+#pragma unroll
+      for (int k = T::P - 1; k >= 0; k--)
+        pe.plane(w, X[k], (u32)k);
+    }
+    c = pe.pos;
   }
   else {
     c = encode_planes256<UInt, T::P>(w, u, maxbits, maxprec);
@@ -103,7 +123,7 @@ __device__ u32 encode_block(W& w, const S* data, const Geom& g, const u32* bc, c
 // decode.c:281-298); returns the block's length in bits
 // ---------------------------------------------------------------------------
 template <typename S, int D, bool STORE>
-__device__ u32 decode_block(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z)
+__device__ __forceinline__ u32 decode_block(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z)
 {
   typedef Traits<S> T;
   typedef typename T::Int Int;
@@ -132,10 +152,36 @@ __device__ u32 decode_block(BitReader& r, S* data, const Geom& g, const u32* bc,
   UInt u[N];
   u32 c;
   if constexpr (D < 4) {
+    PlaneDecoder<N> pd;
+    pd.init(T::P, maxbits, maxprec);
     u64 X[T::P];
-    c = decode_planes<N, T::P>(r, X, maxbits, maxprec);
-    if (STORE)
-      from_planes<N>(X, u);
+    if constexpr (N == 64 && T::P == 64) {
+#pragma unroll
+      for (int k = 63; k >= 32; k--)
+        X[k] = pd.plane(r, (u32)k);
+      const bool low = pd.active() && pd.kmin < 32;
+      if (low) {
+#pragma unroll
+        for (int k = 31; k >= 0; k--)
+          X[k] = pd.plane(r, (u32)k);
+      }
+      if (STORE) {
+#pragma unroll
+        for (int i = 0; i < N; i++)
+          u[i] = 0;
+        coeffs64_half<1>(X, u);
+        if (low)
+          coeffs64_half<0>(X, u);
+      }
+    }
+    else {
+#pragma unroll
+      for (int k = T::P - 1; k >= 0; k--)
+        X[k] = pd.plane(r, (u32)k);
+      if (STORE)
+        from_planes<N>(X, u);
+    }
+    c = pd.used();
   }
   else {
     c = decode_planes256<UInt, T::P>(r, u, maxbits, maxprec);

@@ -516,12 +516,52 @@ ZH_INLINE void from_planes(const u64* X, u32* u)
   }
 }
 
+// Half of the planes of 64 coefficients of 64 bits: H = 1 gives planes 32..63
+// (from the high words), H = 0 planes 0..31, as X[32H + k].  Coding the high
+// half first lets low-rate blocks skip the other transposes entirely.
+template <int H>
+ZH_INLINE void planes64_half(const u64* u, u64* X)
+{
+  u32 a[32], b[32];
+#pragma unroll
+  for (int i = 0; i < 32; i++) {
+    a[i] = (u32)(u[i] >> (32 * H));
+    b[i] = (u32)(u[32 + i] >> (32 * H));
+  }
+  transpose32(a);
+  transpose32(b);
+#pragma unroll
+  for (int k = 0; k < 32; k++)
+    X[32 * H + k] = (u64)a[k] | ((u64)b[k] << 32);
+}
+
+// inverse: coefficient words from the planes of half H (OR-ed into u)
+template <int H>
+ZH_INLINE void coeffs64_half(const u64* X, u64* u)
+{
+  u32 a[32], b[32];
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    a[k] = (u32)X[32 * H + k];
+    b[k] = (u32)(X[32 * H + k] >> 32);
+  }
+  transpose32(a);
+  transpose32(b);
+#pragma unroll
+  for (int i = 0; i < 32; i++) {
+    u[i] |= (u64)a[i] << (32 * H);
+    u[32 + i] |= (u64)b[i] << (32 * H);
+  }
+}
+
 // --------------------------------------------------------------------------
 // bit output: one lane appends to its own bit range [lo, hi) of the stream.
 // Words wholly inside the range are stored; boundary words, shared with a
 // neighbouring block or with bits already in the stream, are OR-ed into
-// memory the host has zeroed beforehand.
+// memory the host has zeroed beforehand.  PLAIN writers (every block range
+// word-aligned) store every word.
 // --------------------------------------------------------------------------
+template <bool PLAIN>
 struct BitWriter {
   u64* out;
   u64 wi;    // index of the word being filled
@@ -535,11 +575,16 @@ struct BitWriter {
   }
   ZH_INLINE void emit(u64 word)
   {
-    u64 b0 = wi << 6;
-    if (b0 >= lo && b0 + 64 <= hi)
+    if (PLAIN) {
       out[wi] = word;
-    else if (word)
-      atomicOr((unsigned long long*)(out + wi), (unsigned long long)word);
+    }
+    else {
+      u64 b0 = wi << 6;
+      if (b0 >= lo && b0 + 64 <= hi)
+        out[wi] = word;
+      else if (word)
+        atomicOr((unsigned long long*)(out + wi), (unsigned long long)word);
+    }
     wi++;
   }
   // append the low n (<= 64) bits of v; v must have no bits at or above n
@@ -574,12 +619,14 @@ struct BitCounter {
   ZH_INLINE void zeros(u64 n) { pos += n; }
 };
 
-// Sequential reader of one lane's bits, with a bound on the words it may load.
+// Sequential reader of one lane's bits with a three-word window (one word of
+// lookahead so that refills do not wait on memory) and a bound on the words
+// it may load.
 struct BitReader {
   const u64* in;
-  u64 nwords;  // readable words
-  u64 next;    // next word index to load
-  u64 w0, w1;  // window; unread bits start at bit `pos` of w0
+  u64 nwords;      // readable words
+  u64 next;        // next word index to load
+  u64 w0, w1, w2;  // unread bits start at bit `pos` of w0
   u32 pos;
 
   ZH_INLINE u64 load(u64 i) const { return i < nwords ? in[i] : 0ull; }
@@ -587,19 +634,16 @@ struct BitReader {
   {
     in = p; nwords = n;
     u64 wi = bit >> 6;
-    w0 = load(wi); w1 = load(wi + 1); next = wi + 2; pos = (u32)(bit & 63);
+    w0 = load(wi); w1 = load(wi + 1); w2 = load(wi + 2); next = wi + 3; pos = (u32)(bit & 63);
   }
-  // next n (<= 64) bits without consuming them
-  ZH_INLINE u64 peek(u32 n) const
-  {
-    u64 v = pos ? ((w0 >> pos) | (w1 << (64 - pos))) : w0;
-    return v & lowmask(n);
-  }
+  // next 64 bits without consuming them
+  ZH_INLINE u64 peek64() const { return pos ? ((w0 >> pos) | (w1 << (64 - pos))) : w0; }
+  ZH_INLINE u64 peek(u32 n) const { return peek64() & lowmask(n); }
   ZH_INLINE void skip(u32 n) // n <= 64
   {
     pos += n;
     if (pos >= 64) {
-      w0 = w1; w1 = load(next++); pos -= 64;
+      w0 = w1; w1 = w2; w2 = load(next++); pos -= 64;
     }
   }
   ZH_INLINE u64 read(u32 n) { u64 v = peek(n); skip(n); return v; }
@@ -611,91 +655,162 @@ struct BitReader {
 };
 
 // --------------------------------------------------------------------------
-// embedded coder for N <= 64 coefficients given as bit planes X[k]
-// (src/template/encode.c:90-256; budget semantics of encode_few_ints:
-// every emitted bit costs one unit and coding stops when the budget is spent;
-// when the worst case fits the budget (codec.c:1-6) there is no budget).
-// Returns the number of bits written.
+// embedded coder for N <= 64 coefficients, one bit plane at a time
+// (src/template/encode.c:90-256).  Plane k of the coefficients is the word
+// X[k] (bit i = bit k of coefficient i).  A plane is coded as the bits of
+// the n coefficients already significant, then group tests over the rest:
+// a 1 when a one-bit remains, followed by the run up to and including that
+// one-bit; the run's last one-bit is implied when it is the last
+// coefficient; a final 0 when none remains.
+//
+// Budget: when the worst case can exceed maxbits (codec.c:1-6) every emitted
+// bit costs one unit and coding stops at zero, which makes the output exactly
+// the first maxbits bits of the unbudgeted code.  The encoder therefore
+// builds each plane's whole code and clips it at the budget.
 // --------------------------------------------------------------------------
 ZH_INLINE bool budget_binds(u32 maxbits, u32 maxprec, u32 size)
 {
   return (maxprec + 1) * size - 1 > maxbits;
 }
 
-template <int N, int P, typename W>
-ZH_INLINE u32 encode_planes(W& w, const u64* X, u32 maxbits, u32 maxprec)
+// bits of a 128-bit value {lo, hi}
+ZH_INLINE void or128(u64& lo, u64& hi, u64 v, u32 s) // v << s, s < 128
 {
-  const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
-  u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
-  const u32 budget0 = budget;
-  u32 n = 0;
-#pragma unroll
-  for (int k = P - 1; k >= 0; k--) {
-    if ((u32)k >= kmin && budget) {
-      u64 x = X[k];
-      u32 m = umin(n, budget);
-      w.put(x & lowmask(m), m);
-      budget -= m;
-      x = shr64(x, n);
-      while (budget && n < (u32)N) {
-        budget--;
-        if (!x) { w.put(0, 1); break; }
-        w.put(1, 1);
-        u32 p = ctz64(x);
-        u32 cnt = (n + p < (u32)N - 1) ? p + 1 : p;
-        if (cnt > budget) { w.put(0, budget); budget = 0; break; }
-        w.put(cnt == p + 1 ? (1ull << p) : 0ull, cnt);
-        budget -= cnt;
-        n += p + 1;
-        x = shr64(x, p + 1);
-      }
-    }
+  if (s < 64) {
+    lo |= v << s;
+    hi |= s ? (v >> (64 - s)) : 0ull;
   }
-  return budget0 - budget;
+  else {
+    hi |= v << (s - 64);
+  }
 }
 
-// decoder mirror (src/template/decode.c:78-278).  Returns bits consumed.
-template <int N, int P>
-ZH_INLINE u32 decode_planes(BitReader& r, u64* X, u32 maxbits, u32 maxprec)
+ZH_INLINE void clear128(u64& lo, u64& hi, u32 s) // clear bit s (s <= 128)
 {
-  const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
-  u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
-  const u32 budget0 = budget;
-  u32 n = 0;
-#pragma unroll
-  for (int k = P - 1; k >= 0; k--) {
-    u64 x = 0;
-    if ((u32)k >= kmin && budget) {
-      u32 m = umin(n, budget);
-      x = r.read(m);
-      budget -= m;
-      while (budget && n < (u32)N) {
-        budget--;
-        if (!r.read(1))
-          break;
-        // scan for the next one-bit; running out of budget or reaching the
-        // last coefficient makes the current coefficient the one
-        u32 av = umin((u32)N - 1 - n, budget);
-        u64 look = r.peek(av);
-        if (look) {
-          u32 p = ctz64(look);
-          r.skip(p + 1);
-          budget -= p + 1;
-          n += p;
-        }
-        else {
-          r.skip(av);
-          budget -= av;
-          n += av;
-        }
-        x |= 1ull << n;
-        n++;
-      }
-    }
-    X[k] = x;
-  }
-  return budget0 - budget;
+  if (s < 64)
+    lo &= ~(1ull << s);
+  else if (s < 128)
+    hi &= ~(1ull << (s - 64));
 }
+
+template <int N>
+struct PlaneEncoder {
+  u32 n;      // coefficients already significant
+  u32 pos;    // bits emitted
+  u32 limit;  // budget (0xffffffff when it cannot bind)
+  u32 kmin;   // lowest plane coded
+
+  ZH_INLINE void init(u32 P, u32 maxbits, u32 maxprec)
+  {
+    n = 0;
+    pos = 0;
+    kmin = P > maxprec ? P - maxprec : 0;
+    limit = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
+  }
+  ZH_INLINE bool active() const { return pos < limit; }
+
+  template <typename W>
+  ZH_INLINE void put_clipped(W& w, u64 v, u32 len)
+  {
+    u32 room = limit - pos;
+    if (len > room) {
+      len = room;
+      v &= lowmask(len);
+    }
+    w.put(v, len);
+    pos += len;
+  }
+
+  template <typename W>
+  ZH_INLINE void plane(W& w, u64 x, u32 k)
+  {
+    if (k < kmin || pos >= limit)
+      return;
+    // verbatim bits of the significant coefficients
+    put_clipped(w, x & lowmask(n), n);
+    if (n >= (u32)N || pos >= limit)
+      return;
+    u64 r = x >> n; // n < N <= 64
+    if (!r) {
+      put_clipped(w, 0, 1);
+      return;
+    }
+    // group-test code of the remainder as a 128-bit word: bit 0 is the first
+    // group test; the j-th one-bit of r (at p) lands at p + j, followed by a
+    // 1 at p + j + 1 when another one-bit follows
+    u64 c0 = 1, c1 = 0;
+    u32 j = 0, p = 0;
+    u64 rr = r;
+    do {
+      p = ctz64(rr);
+      j++;
+      or128(c0, c1, 3ull, p + j);
+      rr &= rr - 1;
+    } while (rr);
+    clear128(c0, c1, p + j + 1);
+    const u32 L = (u32)N - n;
+    u32 clen = (p + 1 < L) ? p + j + 2 : p + j; // the last coefficient's one-bit is implied
+    if (clen > 64) {
+      put_clipped(w, c0, 64);
+      put_clipped(w, c1 & lowmask(clen - 64), clen - 64);
+    }
+    else {
+      put_clipped(w, c0 & lowmask(clen), clen);
+    }
+    n += p + 1;
+  }
+};
+
+// decoder mirror (src/template/decode.c:78-278), with the reference's budget
+// semantics: a scan that runs out of budget, or reaches the last
+// coefficient, takes the current coefficient to be the one-bit.
+template <int N>
+struct PlaneDecoder {
+  u32 n, budget, budget0, kmin;
+
+  ZH_INLINE void init(u32 P, u32 maxbits, u32 maxprec)
+  {
+    n = 0;
+    kmin = P > maxprec ? P - maxprec : 0;
+    budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
+    budget0 = budget;
+  }
+  ZH_INLINE bool active() const { return budget != 0; }
+  ZH_INLINE u32 used() const { return budget0 - budget; }
+
+  ZH_INLINE u64 plane(BitReader& r, u32 k)
+  {
+    if (k < kmin || !budget)
+      return 0;
+    u32 m = umin(n, budget);
+    u64 x = r.read(m);
+    budget -= m;
+    while (budget && n < (u32)N) {
+      u64 w = r.peek64();
+      budget--;
+      if (!(w & 1)) {
+        r.skip(1);
+        break;
+      }
+      u32 av = umin((u32)N - 1 - n, budget); // <= 63
+      u64 look = (w >> 1) & lowmask(av);
+      if (look) {
+        u32 p = ctz64(look);
+        r.skip(p + 2);
+        budget -= p + 1;
+        n += p;
+      }
+      else {
+        r.skip(av + 1);
+        budget -= av;
+        n += av;
+      }
+      x |= 1ull << n;
+      n++;
+    }
+    return x;
+  }
+};
 
 // --------------------------------------------------------------------------
 // 4D blocks (256 coefficients): planes are four 64-bit words; the coder

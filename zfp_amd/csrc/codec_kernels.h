@@ -24,10 +24,12 @@
 
 namespace zfphip {
 
-enum { ENC_FIXED = 0, ENC_OFFSETS = 1, ENC_LENGTH = 2 };
+// ENC_FIXED_PLAIN: fixed rate with every block range word-aligned, so each
+// lane owns whole words and stores them without atomics
+enum { ENC_FIXED = 0, ENC_OFFSETS = 1, ENC_LENGTH = 2, ENC_FIXED_PLAIN = 3 };
 
 template <typename S, int D, int MODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
          u64* __restrict__ out, u64 offset, const u64* __restrict__ boffs, u64* __restrict__ lens)
 {
@@ -43,7 +45,7 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
     return;
   }
   u64 start, end;
-  if (MODE == ENC_FIXED) {
+  if (MODE == ENC_FIXED || MODE == ENC_FIXED_PLAIN) {
     start = offset + (u64)b * z.maxbits;
     end = start + z.maxbits;
   }
@@ -51,7 +53,7 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
     start = offset + boffs[b];
     end = offset + boffs[b + 1];
   }
-  BitWriter w;
+  BitWriter<MODE == ENC_FIXED_PLAIN> w;
   w.begin(out, start, end);
   encode_block<S, D>(w, data, g, bc, z);
   w.finish();
@@ -104,11 +106,16 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
   if (fixed) {
     u64 total = (u64)d.nblocks * z.maxbits;
     u64 w0 = offset >> 6, w1 = (offset + total + 63) >> 6;
-    if ((offset & 63) || (z.maxbits & 63))
+    if ((offset & 63) || (z.maxbits & 63)) {
       hipLaunchKernelGGL(k_prepare, dim3(grid_for(w1 - w0, 256)), dim3(256), 0, st, out, w0, w1,
                          (u32)(offset & 63));
-    hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
-                       d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
+      hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+                         d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
+    }
+    else {
+      hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN>), dim3(grid), dim3(BS), 0, st, (const S*)data,
+                         d.g, d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     *bits_out = total;
