@@ -2,8 +2,8 @@
 codec_block.h), compiled for the host by tests/hostcodec, against the oracle.
 
 This runs the very encode_block/decode_block templates the gfx950 kernels
-instantiate (gather with padding, integer-domain block-floating-point cast,
-lifting, reorder, 32x32 bit-plane transposes, plane coder, bit I/O), one
+instantiate (gather with padding, block-floating-point cast, lifting,
+reorder, 32x32 bit-plane transposes, plane coder, bit I/O), one
 block after another, so that logic errors show up without a GPU.  The GPU
 parity tests remain the proof for the device build itself."""
 import ctypes

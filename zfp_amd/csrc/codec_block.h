@@ -35,8 +35,108 @@ __device__ __forceinline__ void block_coords(u32 b, const Geom& g, u32* c)
 }
 
 // ---------------------------------------------------------------------------
+// embedded coding of the bit planes of N <= 64 coefficients of P bits
+// (src/template/encode.c:90-256): planes from P-1 down, the high half of
+// 64-bit coefficients before the low half, each half a loop over a plane
+// vector with the plane number uniform across the wave.  Returns the bits of
+// the budgeted code.
+// ---------------------------------------------------------------------------
+// The plane loops have no early exit: an exit taken on a wave-wide vote
+// around the lanes' own (divergent) group-test loops makes the compiler
+// treat the plane number as varying across lanes, and index the plane
+// vectors lane by lane.  A plane no lane wants costs a vote and a branch.
+template <int N, int P, typename UInt, typename W>
+__device__ __forceinline__ void encode_half(PlaneEncoder<N>& pe, W& w, const PlaneVec& lo, const PlaneVec& hi,
+                                            u32 base)
+{
+  for (int k = 31; k >= 0; k--) {
+    const bool act = pe.wants(base + (u32)k);
+    if (wave_any(act)) {
+      const int ku = __builtin_amdgcn_readfirstlane(k); // uniform plane number
+      const u64 x = (u64)lo[ku] | (N == 64 ? (u64)hi[ku] << 32 : 0ull);
+      if (act)
+        pe.plane(w, x);
+    }
+  }
+}
+
+template <int N, int P, typename UInt, typename W>
+__device__ __forceinline__ u32 encode_planes(W& w, const UInt* u, u32 maxbits, u32 maxprec)
+{
+  PlaneEncoder<N> pe;
+  pe.init(P, maxbits, maxprec);
+  PlaneVec lo, hi;
+  if constexpr (P == 64) {
+    if (wave_any(pe.wants(63))) {
+      planes_half<N, 1>(u, lo, hi);
+      encode_half<N, P, UInt>(pe, w, lo, hi, 32);
+    }
+  }
+  if (wave_any(pe.wants(31))) {
+    planes_half<N, 0>(u, lo, hi);
+    encode_half<N, P, UInt>(pe, w, lo, hi, 0);
+  }
+  return pe.length();
+}
+
+// decoding mirror (src/template/decode.c:78-278); u receives the
+// coefficients' bits, zero where no plane was coded
+template <int N>
+__device__ __forceinline__ void decode_half(PlaneDecoder<N>& pd, BitReader& r, PlaneVec& lo, PlaneVec& hi,
+                                            u32 base)
+{
+  for (int k = 31; k >= 0; k--) {
+    const bool act = pd.wants(base + (u32)k);
+    u64 x = 0;
+    if (wave_any(act)) {
+      if (act)
+        x = pd.plane(r);
+    }
+    // every plane is written (zero when not coded); the plane number is
+    // uniform, which readfirstlane states to the compiler
+    const int ku = __builtin_amdgcn_readfirstlane(k);
+    lo[ku] = (u32)x;
+    hi[ku] = (u32)(x >> 32);
+  }
+}
+
+template <int N, int P, typename UInt, bool STORE>
+__device__ __forceinline__ u32 decode_planes(BitReader& r, UInt* u, u32 maxbits, u32 maxprec)
+{
+  PlaneDecoder<N> pd;
+  pd.init(P, maxbits, maxprec);
+  u32 w[N];
+  PlaneVec lo, hi;
+  if constexpr (P == 64) {
+    decode_half<N>(pd, r, lo, hi, 32);
+    if (STORE)
+      coeffs_half<N>(lo, hi, w);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      u[i] = STORE ? (UInt)w[i] << (P / 2) : 0;
+  }
+  else {
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      u[i] = 0;
+  }
+  if (wave_any(pd.wants(31))) {
+    decode_half<N>(pd, r, lo, hi, 0);
+    if (STORE) {
+      coeffs_half<N>(lo, hi, w);
+#pragma unroll
+      for (int i = 0; i < N; i++)
+        u[i] |= (UInt)w[i];
+    }
+  }
+  return pd.used();
+}
+
+// ---------------------------------------------------------------------------
 // encode one block (src/template/encodef.c:62-90, encodei.c:6-15,
-// encode.c:259-280); returns the block's length in bits
+// encode.c:259-280); returns the block's length in bits.  A float block
+// with nothing to code (header bit 0) runs the same path with no planes, so
+// that the lanes of a wave do not split on it.
 // ---------------------------------------------------------------------------
 template <typename S, int D, typename W>
 __device__ __forceinline__ u32 encode_block(W& w, const S* data, const Geom& g, const u32* bc, const Params& z)
@@ -51,25 +151,21 @@ __device__ __forceinline__ u32 encode_block(W& w, const S* data, const Geom& g, 
   u32 hdr = 0;
   u32 minbits = z.minbits, maxbits = z.maxbits, maxprec = z.maxprec;
   if constexpr (T::FLOAT) {
-    int emax = block_emax<N>(v);
-    u32 mp = plane_count(emax, z.maxprec, z.minexp, D);
-    u32 e = mp ? (u32)(emax + T::EBIAS) : 0u;
-    if (!e) {
-      // all-zero (or below tolerance) block: a single zero bit
-      w.put(0, 1);
-      if (z.minbits > 1) {
-        w.zeros(z.minbits - 1);
-        return z.minbits;
-      }
-      return 1;
-    }
-    hdr = 1 + T::EBITS;
-    w.put(2 * (u64)e + 1, hdr);
+    const int emax = block_emax<N>(v);
+    const u32 mp = plane_count(emax, z.maxprec, z.minexp, D);
+    const u32 e = mp ? (u32)(emax + T::EBIAS) : 0u;
+    hdr = e ? 1 + T::EBITS : 1;
+    w.put(e ? 2 * (u64)e + 1 : 0ull, hdr);
+    S s;
+    if constexpr (T::P == 64)
+      s = exp2_exact(62 - emax);
+    else
+      s = exp2_exact_f(30 - emax);
 #pragma unroll
     for (int i = 0; i < N; i++)
-      q[i] = fwd_cast_one(v[i], emax);
-    maxprec = mp;
-    minbits = z.minbits - (hdr < z.minbits ? hdr : z.minbits);
+      q[i] = fwd_cast_one(v[i], s);
+    maxprec = e ? mp : 0u;
+    minbits = z.minbits - umin(hdr, z.minbits);
     maxbits = z.maxbits - hdr;
   }
   else {
@@ -83,34 +179,10 @@ __device__ __forceinline__ u32 encode_block(W& w, const S* data, const Geom& g, 
   for (int i = 0; i < N; i++)
     u[i] = ((UInt)q[order_at<D>(i)] + NB<UInt>::mask) ^ NB<UInt>::mask;
   u32 c;
-  if constexpr (D < 4) {
-    PlaneEncoder<N> pe;
-    pe.init(T::P, maxbits, maxprec);
-    u64 X[T::P];
-    if constexpr (N == 64 && T::P == 64) {
-      // high planes first; the low-plane transposes only when still coding
-      planes64_half<1>(u, X);
-#pragma unroll
-      for (int k = 63; k >= 32; k--)
-        pe.plane(w, X[k], (u32)k);
-      if (pe.active() && pe.kmin < 32) {
-        planes64_half<0>(u, X);
-#pragma unroll
-        for (int k = 31; k >= 0; k--)
-          pe.plane(w, X[k], (u32)k);
-      }
-    }
-    else {
-      to_planes<N>(u, X);
-#pragma unroll
-      for (int k = T::P - 1; k >= 0; k--)
-        pe.plane(w, X[k], (u32)k);
-    }
-    c = pe.pos;
-  }
-  else {
+  if constexpr (D < 4)
+    c = encode_planes<N, T::P>(w, u, maxbits, maxprec);
+  else
     c = encode_planes256<UInt, T::P>(w, u, maxbits, maxprec);
-  }
   if (c < minbits) {
     w.zeros(minbits - c);
     c = minbits;
@@ -133,59 +205,26 @@ __device__ __forceinline__ u32 decode_block(BitReader& r, S* data, const Geom& g
   u32 minbits = z.minbits, maxbits = z.maxbits, maxprec = z.maxprec;
   int emax = 0;
   if constexpr (T::FLOAT) {
-    if (!r.read(1)) {
-      if (STORE) {
-        S v[N];
-#pragma unroll
-        for (int i = 0; i < N; i++)
-          v[i] = (S)0;
-        scatter<S, D>(v, data, g, bc);
-      }
-      return z.minbits > 1 ? z.minbits : 1;
+    // header bit 0: a block of zeros, decoded as a block with no planes
+    const bool coded = r.read(1) != 0;
+    if (coded) {
+      emax = (int)r.read(T::EBITS) - T::EBIAS;
+      hdr = 1 + T::EBITS;
+      maxprec = plane_count(emax, z.maxprec, z.minexp, D);
     }
-    emax = (int)r.read(T::EBITS) - T::EBIAS;
-    hdr = 1 + T::EBITS;
-    maxprec = plane_count(emax, z.maxprec, z.minexp, D);
-    minbits = z.minbits - (hdr < z.minbits ? hdr : z.minbits);
+    else {
+      hdr = 1;
+      maxprec = 0;
+    }
+    minbits = z.minbits - umin(hdr, z.minbits);
     maxbits = z.maxbits - hdr;
   }
   UInt u[N];
   u32 c;
-  if constexpr (D < 4) {
-    PlaneDecoder<N> pd;
-    pd.init(T::P, maxbits, maxprec);
-    u64 X[T::P];
-    if constexpr (N == 64 && T::P == 64) {
-#pragma unroll
-      for (int k = 63; k >= 32; k--)
-        X[k] = pd.plane(r, (u32)k);
-      const bool low = pd.active() && pd.kmin < 32;
-      if (low) {
-#pragma unroll
-        for (int k = 31; k >= 0; k--)
-          X[k] = pd.plane(r, (u32)k);
-      }
-      if (STORE) {
-#pragma unroll
-        for (int i = 0; i < N; i++)
-          u[i] = 0;
-        coeffs64_half<1>(X, u);
-        if (low)
-          coeffs64_half<0>(X, u);
-      }
-    }
-    else {
-#pragma unroll
-      for (int k = T::P - 1; k >= 0; k--)
-        X[k] = pd.plane(r, (u32)k);
-      if (STORE)
-        from_planes<N>(X, u);
-    }
-    c = pd.used();
-  }
-  else {
+  if constexpr (D < 4)
+    c = decode_planes<N, T::P, UInt, STORE>(r, u, maxbits, maxprec);
+  else
     c = decode_planes256<UInt, T::P>(r, u, maxbits, maxprec);
-  }
   if (c < minbits)
     c = minbits;
   if (STORE) {

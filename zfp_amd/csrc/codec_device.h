@@ -1,12 +1,16 @@
 // codec_device.h -- device-side building blocks of the zfp block codec for
-// gfx950: gather with partial-block padding, block-floating-point cast in the
-// integer domain, lifting transform, negabinary reorder, bit-plane transpose
-// and the embedded plane coder, plus bit I/O on 64-bit stream words.
+// gfx950: gather with partial-block padding, block-floating-point cast,
+// lifting transform, negabinary reorder, bit-plane transpose and the embedded
+// plane coder, plus bit I/O on 64-bit stream words.
 //
 // Layout: one lane owns one zfp block.  A block's 4^d values live in VGPRs
 // (fully unrolled loops, compile-time indices), so the transform, the reorder
 // and the 32x32 bit-matrix transposes that turn coefficients into bit planes
-// are register renames plus ALU work, never memory traffic.
+// are register renames plus ALU work, never memory traffic.  The plane coder
+// runs as a rolled loop over planes: the plane number is the same in every
+// lane of a wave, so the 32-word plane vectors are indexed through the
+// register-index mode (s_set_gpr_idx) instead of being unrolled, which keeps
+// the kernels small enough for the instruction cache.
 //
 // Every routine cites the reference code whose result it reproduces
 // (LLNL zfp 1.0.1, paths relative to its tree).
@@ -96,14 +100,12 @@ ZH_INLINE u64 shr64(u64 x, u32 n) { return n >= 64 ? 0ull : x >> n; }
 ZH_INLINE u64 shl64(u64 x, u32 n) { return n >= 64 ? 0ull : x << n; }
 ZH_INLINE u32 ctz64(u64 x) { return (u32)__builtin_ctzll(x); }
 ZH_INLINE u32 umin(u32 a, u32 b) { return a < b ? a : b; }
+// true when the predicate holds in any active lane of the wave
+ZH_INLINE bool wave_any(bool p) { return __any(p) != 0; }
 
-// padded source position along one axis of a partial block: the reference
-// pads x, then y, z, w with p1 <- p0, p2 <- p1, p3 <- p0 patterns
-// (src/template/encode.c:8-27), which composes to this per-axis map
-ZH_INLINE int pad_pos(int p, int m)
-{
-  return m >= 4 ? p : m == 3 ? (p == 3 ? 0 : p) : m == 2 ? (p == 2 ? 1 : (p == 3 ? 0 : p)) : 0;
-}
+// 32 words held in VGPRs; an index that is uniform across the wave selects
+// a register without unrolling
+typedef u32 PlaneVec __attribute__((ext_vector_type(32)));
 
 // --------------------------------------------------------------------------
 // block geometry
@@ -113,43 +115,83 @@ struct Geom {
   u32 n[4];   // extents (1 for unused dims)
   u32 nb[4];  // blocks per dim
   u32 dims;
+  // lane order: neighbouring lanes take neighbouring blocks along axis
+  // lax[0] (the one of smallest |stride|), then lax[1], ...; lnb[i] = nb[lax[i]]
+  u32 lax[4];
+  u32 lnb[4];
 };
+
+// The block of lane t in lane order: its coordinates bc (x first) and its
+// position in stream order (x fastest, src/template/compress.c), which fixes
+// where its bits go.  Walking the blocks along the axis of smallest stride
+// makes a wave's loads and stores cover contiguous memory for any axis order
+// of the view, such as a transposed one.
+template <int D>
+ZH_INLINE u32 lane_block(u32 t, const Geom& g, u32* bc)
+{
+  u32 c[4];
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    if (i < D - 1) {
+      c[i] = t % g.lnb[i];
+      t /= g.lnb[i];
+    }
+    else
+      c[i] = t;
+  }
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    u32 x = 0;
+#pragma unroll
+    for (int i = 0; i < D; i++)
+      x = g.lax[i] == (u32)d ? c[i] : x;
+    bc[d] = x;
+  }
+  u32 b = bc[D - 1];
+#pragma unroll
+  for (int d = D - 2; d >= 0; d--)
+    b = b * g.nb[d] + bc[d];
+  return b;
+}
+
+// Element offsets of the four positions of a block along one axis of stride
+// s when only m (1..4) of them lie inside the array.  The reference pads x,
+// then y, z, w with the patterns p1 <- p0 (m = 1), p2 <- p1 and p3 <- p0
+// (m = 2), p3 <- p0 (m = 3) (src/template/encode.c:8-27); per axis these
+// compose to source positions {0,0,0,0}, {0,1,1,0}, {0,1,2,0}, {0,1,2,3}.
+ZH_INLINE void axis_offsets(i64 s, u32 m, i64* o)
+{
+  o[0] = 0;
+  o[1] = m >= 2 ? s : 0;
+  o[2] = m >= 3 ? 2 * s : (m == 2 ? s : 0);
+  o[3] = m >= 4 ? 3 * s : 0;
+}
 
 // Gather the 4^D values of block (bx,by,bz,bw) into v[] with partial-block
 // padding (encode1.c:4-20, encode2.c:4-25, encode3.c:4-31, encode4.c:4-38).
+// Whole and partial blocks take the same path: padding only changes the
+// offset tables.
 template <typename S, int D>
 ZH_INLINE void gather(S* v, const S* data, const Geom& g, const u32* b)
 {
   constexpr int N = 1 << (2 * D);
-  int m[4];
-  bool full = true;
+  i64 o[4][4];
   const S* p = data;
 #pragma unroll
   for (int d = 0; d < D; d++) {
-    u32 o = 4 * b[d];
-    m[d] = (int)umin(4u, g.n[d] - o);
-    full = full && m[d] == 4;
-    p += (i64)o * g.s[d];
+    const u32 c = 4 * b[d];
+    p += (i64)c * g.s[d];
+    axis_offsets(g.s[d], umin(4u, g.n[d] - c), o[d]);
   }
-  if (full) {
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-      i64 off = (i64)(i & 3) * g.s[0];
-      if (D > 1) off += (i64)((i >> 2) & 3) * g.s[1];
-      if (D > 2) off += (i64)((i >> 4) & 3) * g.s[2];
-      if (D > 3) off += (i64)((i >> 6) & 3) * g.s[3];
-      v[i] = p[off];
-    }
-  }
-  else {
+  for (int r = 0; r < N / 4; r++) {
+    const S* q = p;
+    if (D > 1) q += o[1][r & 3];
+    if (D > 2) q += o[2][(r >> 2) & 3];
+    if (D > 3) q += o[3][r >> 4];
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-      i64 off = (i64)pad_pos(i & 3, m[0]) * g.s[0];
-      if (D > 1) off += (i64)pad_pos((i >> 2) & 3, m[1]) * g.s[1];
-      if (D > 2) off += (i64)pad_pos((i >> 4) & 3, m[2]) * g.s[2];
-      if (D > 3) off += (i64)pad_pos((i >> 6) & 3, m[3]) * g.s[3];
-      v[i] = p[off];
-    }
+    for (int x = 0; x < 4; x++)
+      v[4 * r + x] = q[o[0][x]];
   }
 }
 
@@ -167,15 +209,17 @@ ZH_INLINE void scatter(const S* v, S* data, const Geom& g, const u32* b)
     p += (i64)o * g.s[d];
   }
 #pragma unroll
-  for (int i = 0; i < N; i++) {
-    u32 x = i & 3, y = (i >> 2) & 3, z = (i >> 4) & 3, w = (i >> 6) & 3;
-    bool ok = x < m[0];
-    i64 off = (i64)x * g.s[0];
-    if (D > 1) { ok = ok && y < m[1]; off += (i64)y * g.s[1]; }
-    if (D > 2) { ok = ok && z < m[2]; off += (i64)z * g.s[2]; }
-    if (D > 3) { ok = ok && w < m[3]; off += (i64)w * g.s[3]; }
-    if (ok)
-      p[off] = v[i];
+  for (int r = 0; r < N / 4; r++) {
+    const u32 y = r & 3, z = (r >> 2) & 3, w = r >> 4;
+    bool ok = true;
+    S* q = p;
+    if (D > 1) { ok = ok && y < m[1]; q += (i64)y * g.s[1]; }
+    if (D > 2) { ok = ok && z < m[2]; q += (i64)z * g.s[2]; }
+    if (D > 3) { ok = ok && w < m[3]; q += (i64)w * g.s[3]; }
+#pragma unroll
+    for (u32 x = 0; x < 4; x++)
+      if (ok && x < m[0])
+        q[(i64)x * g.s[0]] = v[4 * r + x];
   }
 }
 
@@ -223,54 +267,12 @@ ZH_INLINE int block_emax(const float* v)
   return ef == 0 ? -126 : ef - 126;
 }
 
-// (Int)(2^(P-2-emax) * x) with truncation, computed exactly on the bits.
-// NaN, infinities, out-of-range products and an infinite scale factor
-// (emax < -961 for double, < -97 for float) all give INT_MIN, the value the
-// reference's conversion produces on x86-64 (src/template/encodef.c:42-59).
-ZH_INLINE i64 fwd_cast_one(double x, int emax)
-{
-  u64 bits = (u64)__double_as_longlong(x);
-  int ef = (int)((bits >> 52) & 0x7ff);
-  u64 man = bits & 0x000fffffffffffffull;
-  if (ef == 2047 || emax < -961)
-    return (i64)0x8000000000000000ull;
-  u64 m = ef ? (man | (1ull << 52)) : man;
-  int sh = (ef ? ef : 1) - 1013 - emax;
-  u64 mag;
-  if (sh >= 0) {
-    // m << sh must stay below 2^63
-    if (m && (64 - __builtin_clzll(m)) + sh > 63)
-      return (i64)0x8000000000000000ull;
-    mag = m << sh;
-  }
-  else
-    mag = shr64(m, (u32)(-sh));
-  return (bits >> 63) ? -(i64)mag : (i64)mag;
-}
-
-ZH_INLINE i32 fwd_cast_one(float x, int emax)
-{
-  u32 bits = __float_as_uint(x);
-  int ef = (int)((bits >> 23) & 0xff);
-  u32 man = bits & 0x007fffffu;
-  if (ef == 255 || emax < -97)
-    return (i32)0x80000000u;
-  u32 m = ef ? (man | (1u << 23)) : man;
-  int sh = (ef ? ef : 1) - 120 - emax;
-  u32 mag;
-  if (sh >= 0) {
-    if (m && (32 - __builtin_clz(m)) + sh > 31)
-      return (i32)0x80000000u;
-    mag = m << sh;
-  }
-  else
-    mag = (-sh) >= 32 ? 0u : (m >> (-sh));
-  return (bits >> 31) ? -(i32)mag : (i32)mag;
-}
-
-// 2^e as a double, exactly as ldexp(1.0, e) rounds it (0 below 2^-1074)
+// 2^e as a double, exactly as ldexp(1.0, e) rounds it (0 below 2^-1074,
+// +inf from 2^1024 up)
 ZH_INLINE double exp2_exact(int e)
 {
+  if (e >= 1024)
+    return __longlong_as_double((long long)0x7ff0000000000000ull);
   if (e >= -1022)
     return __longlong_as_double((long long)((u64)(e + 1023) << 52));
   if (e >= -1074)
@@ -280,11 +282,32 @@ ZH_INLINE double exp2_exact(int e)
 
 ZH_INLINE float exp2_exact_f(int e)
 {
+  if (e >= 128)
+    return __uint_as_float(0x7f800000u);
   if (e >= -126)
     return __uint_as_float((u32)(e + 127) << 23);
   if (e >= -149)
     return __uint_as_float(1u << (e + 149));
   return 0.0f;
+}
+
+// (Int)(s * x) with s = 2^(P-2-emax) (src/template/encodef.c:42-59).  The
+// product is exact: a power-of-two scale of a value below 2^emax in
+// magnitude, rounded only where it falls below 1 and truncates to 0 anyway.
+// What the reference's conversion yields on x86-64 for a product outside
+// the integer range, NaN (inf * 0) or infinity (inf inputs, or an infinite
+// scale when emax < -961 for double, < -97 for float) is INT_MIN; the range
+// test is false for NaN, so all of these take that value.
+ZH_INLINE i64 fwd_cast_one(double x, double s)
+{
+  double y = s * x;
+  return __builtin_fabs(y) < 0x1p63 ? (i64)y : (i64)0x8000000000000000ull;
+}
+
+ZH_INLINE i32 fwd_cast_one(float x, float s)
+{
+  float y = s * x;
+  return __builtin_fabsf(y) < 0x1p31f ? (i32)y : (i32)0x80000000u;
 }
 
 // (Scalar)(2^(emax-(P-2)) * (Scalar)i): round-to-nearest-even conversion,
@@ -352,6 +375,11 @@ ZH_INLINE void inv_xform(Int* q)
 // bit-plane transpose: 32x32 bit matrices, least significant bit = column 0.
 // After transpose32(a), bit i of a[j] is bit j of the original a[i].
 // --------------------------------------------------------------------------
+// Stage J swaps the J x J off-diagonal sub-blocks of every 2J x 2J block:
+// the bits of a[k] selected by ~M, moved down by J, trade places with the
+// bits of a[k + J] selected by M.  Stages 16 and 8 move whole bytes (one
+// byte permute per word on gfx950); the others are a shift and a bitfield
+// insert per word.
 template <int J, u32 M>
 ZH_INLINE void tp_stage(u32* a)
 {
@@ -359,9 +387,9 @@ ZH_INLINE void tp_stage(u32* a)
   for (int k = 0; k < 32; k++) {
     if (k & J)
       continue;
-    u32 t = ((a[k] >> J) ^ a[k + J]) & M;
-    a[k + J] ^= t;
-    a[k] ^= t << J;
+    const u32 lo = a[k], hi = a[k + J];
+    a[k] = (lo & M) | ((hi << J) & ~M);
+    a[k + J] = ((lo >> J) & M) | (hi & ~M);
   }
 }
 
@@ -374,192 +402,97 @@ ZH_INLINE void transpose32(u32* a)
   tp_stage<1, 0x55555555u>(a);
 }
 
-// Planes of N <= 64 coefficients: X[k] bit i = bit k of u[i].
-template <int N>
-ZH_INLINE void to_planes(const u64* u, u64* X)
+// Planes 32H .. 32H+31 of N <= 64 coefficients: bit i of lo[k] is bit
+// 32H + k of coefficient i, bit i of hi[k] that of coefficient 32 + i (hi is
+// zero unless N = 64).  64-bit coefficients have two halves of planes; the
+// coder takes the high half first, so blocks whose budget runs out there
+// never transpose the low half.
+template <int N, int H, typename UInt>
+ZH_INLINE void planes_half(const UInt* u, PlaneVec& lo, PlaneVec& hi)
 {
-  // 64-bit coefficients: four (or two, or direct) 32x32 transposes
-  if (N == 4) {
+  static_assert(H == 0 || sizeof(UInt) == 8, "32-bit coefficients have one half");
+  if constexpr (N == 4) {
 #pragma unroll
-    for (int k = 0; k < 64; k++) {
-      u64 x = 0;
+    for (int k = 0; k < 32; k++) {
+      u32 x = 0;
 #pragma unroll
       for (int i = 0; i < 4; i++)
-        x |= ((u[i] >> k) & 1ull) << i;
-      X[k] = x;
+        x |= (u32)((u[i] >> (32 * H + k)) & 1u) << i;
+      lo[k] = x;
+      hi[k] = 0;
     }
-    return;
   }
-  u32 a[32], b[32];
-  constexpr int H = N >= 32 ? 32 : N;
-#pragma unroll
-  for (int half = 0; half < 2; half++) { // half 0: bits 0..31, half 1: bits 32..63
+  else {
+    u32 a[32];
 #pragma unroll
     for (int i = 0; i < 32; i++)
-      a[i] = i < H ? (u32)(u[i] >> (32 * half)) : 0u;
+      a[i] = i < N ? (u32)(u[i] >> (32 * H)) : 0u;
     transpose32(a);
-    if (N == 64) {
-#pragma unroll
-      for (int i = 0; i < 32; i++)
-        b[i] = (u32)(u[32 + i] >> (32 * half));
-      transpose32(b);
-    }
 #pragma unroll
     for (int k = 0; k < 32; k++)
-      X[32 * half + k] = (u64)a[k] | (N == 64 ? ((u64)b[k] << 32) : 0ull);
-  }
-}
-
-template <int N>
-ZH_INLINE void to_planes(const u32* u, u64* X)
-{
-  if (N == 4) {
-#pragma unroll
-    for (int k = 0; k < 32; k++) {
-      u64 x = 0;
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-        x |= (u64)((u[i] >> k) & 1u) << i;
-      X[k] = x;
-    }
-    return;
-  }
-  u32 a[32], b[32];
-  constexpr int H = N >= 32 ? 32 : N;
-#pragma unroll
-  for (int i = 0; i < 32; i++)
-    a[i] = i < H ? u[i] : 0u;
-  transpose32(a);
-  if (N == 64) {
-#pragma unroll
-    for (int i = 0; i < 32; i++)
-      b[i] = u[32 + i];
-    transpose32(b);
-  }
-#pragma unroll
-  for (int k = 0; k < 32; k++)
-    X[k] = (u64)a[k] | (N == 64 ? ((u64)b[k] << 32) : 0ull);
-}
-
-// inverse of to_planes
-template <int N>
-ZH_INLINE void from_planes(const u64* X, u64* u)
-{
-  if (N == 4) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      u64 v = 0;
-#pragma unroll
-      for (int k = 0; k < 64; k++)
-        v |= ((X[k] >> i) & 1ull) << k;
-      u[i] = v;
-    }
-    return;
-  }
-  u32 a[32], b[32];
-  constexpr int H = N >= 32 ? 32 : N;
-#pragma unroll
-  for (int i = 0; i < N; i++)
-    u[i] = 0;
-#pragma unroll
-  for (int half = 0; half < 2; half++) {
-#pragma unroll
-    for (int k = 0; k < 32; k++) {
-      a[k] = (u32)X[32 * half + k];
-      b[k] = (u32)(X[32 * half + k] >> 32);
-    }
-    transpose32(a);
-    if (N == 64)
-      transpose32(b);
-#pragma unroll
-    for (int i = 0; i < H; i++)
-      u[i] |= (u64)a[i] << (32 * half);
-    if (N == 64) {
+      lo[k] = a[k];
+    if constexpr (N == 64) {
 #pragma unroll
       for (int i = 0; i < 32; i++)
-        u[32 + i] |= (u64)b[i] << (32 * half);
-    }
-  }
-}
-
-template <int N>
-ZH_INLINE void from_planes(const u64* X, u32* u)
-{
-  if (N == 4) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      u32 v = 0;
+        a[i] = (u32)(u[32 + i] >> (32 * H));
+      transpose32(a);
 #pragma unroll
       for (int k = 0; k < 32; k++)
-        v |= (u32)((X[k] >> i) & 1ull) << k;
-      u[i] = v;
+        hi[k] = a[k];
     }
-    return;
-  }
-  u32 a[32], b[32];
-  constexpr int H = N >= 32 ? 32 : N;
+    else {
 #pragma unroll
-  for (int k = 0; k < 32; k++) {
-    a[k] = (u32)X[k];
-    b[k] = (u32)(X[k] >> 32);
-  }
-  transpose32(a);
-  if (N == 64)
-    transpose32(b);
-#pragma unroll
-  for (int i = 0; i < H; i++)
-    u[i] = a[i];
-  if (N == 64) {
-#pragma unroll
-    for (int i = 0; i < 32; i++)
-      u[32 + i] = b[i];
+      for (int k = 0; k < 32; k++)
+        hi[k] = 0;
+    }
   }
 }
 
-// Half of the planes of 64 coefficients of 64 bits: H = 1 gives planes 32..63
-// (from the high words), H = 0 planes 0..31, as X[32H + k].  Coding the high
-// half first lets low-rate blocks skip the other transposes entirely.
-template <int H>
-ZH_INLINE void planes64_half(const u64* u, u64* X)
+// inverse of planes_half: w[i] receives the 32 bits of coefficient i that
+// one half of the planes holds (kept apart from the other half so that a
+// 64-bit coefficient is not a register pair before both halves exist)
+template <int N>
+ZH_INLINE void coeffs_half(const PlaneVec& lo, const PlaneVec& hi, u32* w)
 {
-  u32 a[32], b[32];
+  if constexpr (N == 4) {
 #pragma unroll
-  for (int i = 0; i < 32; i++) {
-    a[i] = (u32)(u[i] >> (32 * H));
-    b[i] = (u32)(u[32 + i] >> (32 * H));
+    for (int i = 0; i < 4; i++) {
+      u32 x = 0;
+#pragma unroll
+      for (int k = 0; k < 32; k++)
+        x |= ((lo[k] >> i) & 1u) << k;
+      w[i] = x;
+    }
   }
-  transpose32(a);
-  transpose32(b);
+  else {
+    u32 a[32];
 #pragma unroll
-  for (int k = 0; k < 32; k++)
-    X[32 * H + k] = (u64)a[k] | ((u64)b[k] << 32);
-}
-
-// inverse: coefficient words from the planes of half H (OR-ed into u)
-template <int H>
-ZH_INLINE void coeffs64_half(const u64* X, u64* u)
-{
-  u32 a[32], b[32];
+    for (int k = 0; k < 32; k++)
+      a[k] = lo[k];
+    transpose32(a);
 #pragma unroll
-  for (int k = 0; k < 32; k++) {
-    a[k] = (u32)X[32 * H + k];
-    b[k] = (u32)(X[32 * H + k] >> 32);
-  }
-  transpose32(a);
-  transpose32(b);
+    for (int i = 0; i < (N < 32 ? N : 32); i++)
+      w[i] = a[i];
+    if constexpr (N == 64) {
 #pragma unroll
-  for (int i = 0; i < 32; i++) {
-    u[i] |= (u64)a[i] << (32 * H);
-    u[32 + i] |= (u64)b[i] << (32 * H);
+      for (int k = 0; k < 32; k++)
+        a[k] = hi[k];
+      transpose32(a);
+#pragma unroll
+      for (int i = 0; i < 32; i++)
+        w[32 + i] = a[i];
+    }
   }
 }
 
 // --------------------------------------------------------------------------
 // bit output: one lane appends to its own bit range [lo, hi) of the stream.
-// Words wholly inside the range are stored; boundary words, shared with a
-// neighbouring block or with bits already in the stream, are OR-ed into
-// memory the host has zeroed beforehand.  PLAIN writers (every block range
-// word-aligned) store every word.
+// Bits appended past hi are dropped, so a coder may write a budgeted block's
+// unbudgeted code and let the range cut it (the budgeted code is a prefix of
+// the unbudgeted one).  Words wholly inside the range are stored; boundary
+// words, shared with a neighbouring block or with bits already in the
+// stream, are OR-ed into memory the host has zeroed beforehand.  PLAIN
+// writers (every block range whole words) store every word they own.
 // --------------------------------------------------------------------------
 template <bool PLAIN>
 struct BitWriter {
@@ -575,11 +508,14 @@ struct BitWriter {
   }
   ZH_INLINE void emit(u64 word)
   {
+    const u64 b0 = wi << 6;
     if (PLAIN) {
-      out[wi] = word;
+      if (b0 < hi)
+        out[wi] = word;
     }
-    else {
-      u64 b0 = wi << 6;
+    else if (b0 < hi) {
+      if (b0 + 64 > hi)
+        word &= lowmask((u32)(hi - b0));
       if (b0 >= lo && b0 + 64 <= hi)
         out[wi] = word;
       else if (word)
@@ -594,7 +530,7 @@ struct BitWriter {
     u32 nf = fill + n;
     if (nf >= 64) {
       emit(acc);
-      acc = fill ? (v >> (64 - fill)) : 0ull;
+      acc = (v >> 1) >> (63 - fill); // bits of v that did not fit (none when fill = 0)
       nf -= 64;
     }
     fill = nf;
@@ -637,7 +573,7 @@ struct BitReader {
     w0 = load(wi); w1 = load(wi + 1); w2 = load(wi + 2); next = wi + 3; pos = (u32)(bit & 63);
   }
   // next 64 bits without consuming them
-  ZH_INLINE u64 peek64() const { return pos ? ((w0 >> pos) | (w1 << (64 - pos))) : w0; }
+  ZH_INLINE u64 peek64() const { return (w0 >> pos) | ((w1 << 1) << (63 - pos)); }
   ZH_INLINE u64 peek(u32 n) const { return peek64() & lowmask(n); }
   ZH_INLINE void skip(u32 n) // n <= 64
   {
@@ -666,37 +602,30 @@ struct BitReader {
 // Budget: when the worst case can exceed maxbits (codec.c:1-6) every emitted
 // bit costs one unit and coding stops at zero, which makes the output exactly
 // the first maxbits bits of the unbudgeted code.  The encoder therefore
-// builds each plane's whole code and clips it at the budget.
+// writes each plane's whole code, stops after the plane that reaches the
+// budget, and leaves the cut to the writer's range.
 // --------------------------------------------------------------------------
 ZH_INLINE bool budget_binds(u32 maxbits, u32 maxprec, u32 size)
 {
   return (maxprec + 1) * size - 1 > maxbits;
 }
 
-// bits of a 128-bit value {lo, hi}
-ZH_INLINE void or128(u64& lo, u64& hi, u64 v, u32 s) // v << s, s < 128
+// {c0, c1} ^= v << s for a 128-bit value, v < 4, s < 128 (a bit pushed
+// past position 127 is dropped)
+ZH_INLINE void xor128(u64& c0, u64& c1, u64 v, u32 s)
 {
-  if (s < 64) {
-    lo |= v << s;
-    hi |= s ? (v >> (64 - s)) : 0ull;
-  }
-  else {
-    hi |= v << (s - 64);
-  }
-}
-
-ZH_INLINE void clear128(u64& lo, u64& hi, u32 s) // clear bit s (s <= 128)
-{
-  if (s < 64)
-    lo &= ~(1ull << s);
-  else if (s < 128)
-    hi &= ~(1ull << (s - 64));
+  const u32 t = s & 63;
+  const u64 a = v << t;
+  const u64 spill = (v >> 1) >> (63 - t);
+  const bool up = s >= 64;
+  c0 ^= up ? 0ull : a;
+  c1 ^= up ? a : spill;
 }
 
 template <int N>
 struct PlaneEncoder {
   u32 n;      // coefficients already significant
-  u32 pos;    // bits emitted
+  u32 pos;    // bits of unbudgeted code written
   u32 limit;  // budget (0xffffffff when it cannot bind)
   u32 kmin;   // lowest plane coded
 
@@ -707,57 +636,59 @@ struct PlaneEncoder {
     kmin = P > maxprec ? P - maxprec : 0;
     limit = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
   }
-  ZH_INLINE bool active() const { return pos < limit; }
+  // whether plane k is coded: within the precision and the budget not spent
+  ZH_INLINE bool wants(u32 k) const { return k >= kmin && pos < limit; }
+  // bits of the budgeted code
+  ZH_INLINE u32 length() const { return umin(pos, limit); }
 
+  // one plane (bit i of x = bit of coefficient i): the bits of the n
+  // coefficients already significant, then the group-test code of the rest
   template <typename W>
-  ZH_INLINE void put_clipped(W& w, u64 v, u32 len)
+  ZH_INLINE void plane(W& w, u64 x)
   {
-    u32 room = limit - pos;
-    if (len > room) {
-      len = room;
-      v &= lowmask(len);
-    }
-    w.put(v, len);
-    pos += len;
-  }
-
-  template <typename W>
-  ZH_INLINE void plane(W& w, u64 x, u32 k)
-  {
-    if (k < kmin || pos >= limit)
-      return;
-    // verbatim bits of the significant coefficients
-    put_clipped(w, x & lowmask(n), n);
-    if (n >= (u32)N || pos >= limit)
-      return;
-    u64 r = x >> n; // n < N <= 64
-    if (!r) {
-      put_clipped(w, 0, 1);
+    if (n >= (u32)N) {
+      // every coefficient significant: the plane verbatim
+      w.put(x, (u32)N);
+      pos += (u32)N;
       return;
     }
-    // group-test code of the remainder as a 128-bit word: bit 0 is the first
-    // group test; the j-th one-bit of r (at p) lands at p + j, followed by a
-    // 1 at p + j + 1 when another one-bit follows
-    u64 c0 = 1, c1 = 0;
+    const u64 r = x >> n; // n < N <= 64
+    // group-test code of the remainder as a 128-bit word: bit 0 is the
+    // first group test; the j-th one-bit of r (at p) lands at p + j,
+    // followed by a 1 at p + j + 1 (the next group test)
+    u64 c0 = r ? 1ull : 0ull, c1 = 0;
     u32 j = 0, p = 0;
-    u64 rr = r;
-    do {
+    for (u64 rr = r; rr; rr &= rr - 1) {
       p = ctz64(rr);
       j++;
-      or128(c0, c1, 3ull, p + j);
-      rr &= rr - 1;
-    } while (rr);
-    clear128(c0, c1, p + j + 1);
-    const u32 L = (u32)N - n;
-    u32 clen = (p + 1 < L) ? p + j + 2 : p + j; // the last coefficient's one-bit is implied
-    if (clen > 64) {
-      put_clipped(w, c0, 64);
-      put_clipped(w, c1 & lowmask(clen - 64), clen - 64);
+      xor128(c0, c1, 3ull, p + j);
     }
-    else {
-      put_clipped(w, c0 & lowmask(clen), clen);
+    u32 clen = 1; // a remainder without one-bits: a single 0 group test
+    const u32 n0 = n;
+    if (r) {
+      if (p + 1 < (u32)N - n) {
+        // the group test after the last one-bit is a 0
+        xor128(c0, c1, 1ull, p + j + 1);
+        clen = p + j + 2;
+      }
+      else {
+        // the one-bit of the last coefficient is implied: neither it nor
+        // a group test follows the run
+        xor128(c0, c1, 3ull, p + j);
+        clen = p + j;
+      }
+      n += p + 1;
     }
-    n += p + 1;
+    // the plane's code: the n0 verbatim bits, then the clen group-test
+    // bits, appended as up to three words
+    const u32 L = n0 + clen;
+    w.put((x & lowmask(n0)) | (c0 << n0), umin(L, 64u));
+    if (L > 64) {
+      w.put(((c0 >> 1) >> (63 - n0)) | (c1 << n0), umin(L - 64, 64u));
+      if (L > 128)
+        w.put((c1 >> 1) >> (63 - n0), L - 128);
+    }
+    pos += L;
   }
 };
 
@@ -775,36 +706,48 @@ struct PlaneDecoder {
     budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
     budget0 = budget;
   }
-  ZH_INLINE bool active() const { return budget != 0; }
+  ZH_INLINE bool wants(u32 k) const { return k >= kmin && budget != 0; }
   ZH_INLINE u32 used() const { return budget0 - budget; }
 
-  ZH_INLINE u64 plane(BitReader& r, u32 k)
+  // one plane; the caller checks wants(k)
+  ZH_INLINE u64 plane(BitReader& r)
   {
-    if (k < kmin || !budget)
-      return 0;
-    u32 m = umin(n, budget);
-    u64 x = r.read(m);
-    budget -= m;
+    // verbatim bits of the significant coefficients; when the group test
+    // that follows them is a 0 (the usual end of a plane) it is taken in
+    // the same step
+    const u32 m = umin(n, budget);
+    u64 x;
+    if (m < 64) {
+      const u64 v = r.peek64();
+      x = v & lowmask(m);
+      const bool stop = n < (u32)N && budget > m && !((v >> m) & 1);
+      const u32 take = m + (stop ? 1u : 0u);
+      r.skip(take);
+      budget -= take;
+      if (stop)
+        return x;
+    }
+    else {
+      x = r.peek64();
+      r.skip(64);
+      budget -= 64;
+    }
+    // group tests: a 1, then the run of zeros up to the next one-bit; a
+    // scan that reaches the last coefficient or the end of the budget
+    // stops there and that coefficient takes the one-bit
     while (budget && n < (u32)N) {
-      u64 w = r.peek64();
+      const u64 w = r.peek64();
       budget--;
       if (!(w & 1)) {
         r.skip(1);
         break;
       }
-      u32 av = umin((u32)N - 1 - n, budget); // <= 63
-      u64 look = (w >> 1) & lowmask(av);
-      if (look) {
-        u32 p = ctz64(look);
-        r.skip(p + 2);
-        budget -= p + 1;
-        n += p;
-      }
-      else {
-        r.skip(av + 1);
-        budget -= av;
-        n += av;
-      }
+      const u32 av = umin((u32)N - 1 - n, budget); // <= 63
+      const u32 p = ctz64(((w >> 1) & lowmask(av)) | (1ull << av));
+      const u32 hit = p < av ? 1u : 0u; // the run ends in a read one-bit
+      r.skip(1 + p + hit);
+      budget -= p + hit;
+      n += p;
       x |= 1ull << n;
       n++;
     }

@@ -3,7 +3,8 @@
 // per dimensionality (inst_d1.hip .. inst_d4.hip) so that the 1D-4D kernel
 // families compile in parallel.
 //
-// Kernels (one lane = one zfp block, blocks in stream order):
+// Kernels (one lane = one zfp block; lanes walk the blocks along the axis of
+// smallest stride, each block's bits go to its place in stream order):
 //   k_encode<S,D,MODE>  gather -> BFP cast -> lifting -> reorder -> planes ->
 //                       embedded coder, writing the block's bits at its offset
 //                       (fixed rate: off + b*maxbits; otherwise from a scan)
@@ -33,11 +34,11 @@ __global__ void __launch_bounds__(256, 2)
 k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
          u64* __restrict__ out, u64 offset, const u64* __restrict__ boffs, u64* __restrict__ lens)
 {
-  u32 b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nblocks)
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblocks)
     return;
   u32 bc[4] = {0, 0, 0, 0};
-  block_coords<D>(b, g, bc);
+  const u32 b = lane_block<D>(t, g, bc);
   if (MODE == ENC_LENGTH) {
     BitCounter cnt;
     cnt.pos = 0;
@@ -59,19 +60,47 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
   w.finish();
 }
 
-template <typename S, int D>
+// STAGE (fixed rate, blocks of at most `stride` words): each lane first
+// copies its block's words into its own slot of LDS, with all loads in
+// flight at once, and the bit reader then refills from LDS.  Reading the
+// stream word by word from global memory inside the coder loop would wait
+// out the full memory latency at every refill.
+template <typename S, int D, bool STAGE>
 __global__ void __launch_bounds__(256)
 k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restrict__ in,
-         u64 nwords, u64 offset, const u64* __restrict__ boffs)
+         u64 nwords, u64 offset, const u64* __restrict__ boffs, u32 stride)
 {
-  u32 b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nblocks)
+  extern __shared__ u64 stage[];
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblocks)
     return;
   u32 bc[4] = {0, 0, 0, 0};
-  block_coords<D>(b, g, bc);
-  u64 start = offset + (boffs ? boffs[b] : (u64)b * z.maxbits);
+  const u32 b = lane_block<D>(t, g, bc);
+  const u64 start = offset + (boffs ? boffs[b] : (u64)b * z.maxbits);
   BitReader r;
-  r.begin(in, nwords, start);
+  if constexpr (STAGE) {
+    const u64 first = start >> 6;
+    const u32 cnt = (u32)(((start + z.maxbits + 63) >> 6) - first); // <= stride
+    u64* slot = stage + (u64)threadIdx.x * stride;
+    // batches of 16 loads with clamped (always valid) addresses, then the
+    // stores of the words that exist; the host guarantees nwords >= 1
+    for (u32 j0 = 0; j0 < cnt; j0 += 16) {
+      u64 v[16];
+#pragma unroll
+      for (u32 q = 0; q < 16; q++) {
+        const u64 i = first + j0 + q;
+        v[q] = in[i < nwords ? i : nwords - 1];
+      }
+#pragma unroll
+      for (u32 q = 0; q < 16; q++)
+        if (j0 + q < cnt)
+          slot[j0 + q] = first + j0 + q < nwords ? v[q] : 0ull;
+    }
+    r.begin(slot, cnt, start & 63);
+  }
+  else {
+    r.begin(in, nwords, start);
+  }
   decode_block<S, D, true>(r, data, g, bc, z);
 }
 
@@ -184,8 +213,22 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
     HIPCHK(hipGetLastError());
     boffs = (const u64*)lp;
   }
-  hipLaunchKernelGGL((k_decode<S, D>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z, in,
-                     nwords, offset, fixed ? (const u64*)nullptr : boffs);
+  // fixed rate: stage blocks through LDS when a workgroup's slots fit in
+  // 64 KiB (an odd slot stride spreads the lanes over the LDS banks)
+  const u32 cmax = (u32)((z.maxbits + 63) / 64) + ((offset & 63) || (z.maxbits & 63) ? 1u : 0u);
+  const u32 stride = cmax | 1u;
+  unsigned sbs = 256;
+  while (sbs > 64 && (u64)sbs * stride * 8 > 65536)
+    sbs /= 2;
+  if (fixed && nwords > 0 && (u64)sbs * stride * 8 <= 65536) {
+    hipLaunchKernelGGL((k_decode<S, D, true>), dim3(grid_for(d.nblocks, sbs)), dim3(sbs),
+                       (size_t)sbs * stride * 8, st, (S*)data, d.g, d.nblocks, z, in, nwords, offset,
+                       (const u64*)nullptr, stride);
+  }
+  else {
+    hipLaunchKernelGGL((k_decode<S, D, false>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z, in,
+                       nwords, offset, fixed ? (const u64*)nullptr : boffs, 0u);
+  }
   HIPCHK(hipGetLastError());
   if (fixed)
     total = (u64)d.nblocks * z.maxbits;

@@ -78,6 +78,18 @@ int make_desc(int type, const size_t* n, const ptrdiff_t* s, Desc& d)
   if (nb > 0xffffffffull)
     return record_error(ZFP_HIP_EINVAL, "more than 2^32-1 blocks");
   d.g.dims = (u32)d.dims;
+  // lane order: axes by increasing |stride| (stable), unused axes last
+  for (int i = 0; i < 4; i++)
+    d.g.lax[i] = (u32)i;
+  for (int i = 1; i < d.dims; i++)
+    for (int j = i; j > 0; j--) {
+      i64 a = d.g.s[d.g.lax[j]], b = d.g.s[d.g.lax[j - 1]];
+      if ((a < 0 ? -a : a) < (b < 0 ? -b : b)) {
+        u32 t = d.g.lax[j]; d.g.lax[j] = d.g.lax[j - 1]; d.g.lax[j - 1] = t;
+      }
+    }
+  for (int i = 0; i < 4; i++)
+    d.g.lnb[i] = d.g.nb[d.g.lax[i]];
   d.nblocks = (u32)nb;
   return ZFP_HIP_OK;
 }
