@@ -91,6 +91,55 @@ int zfp_hip_index_export(const void* stream_begin, uint64_t* offsets, size_t cou
 int zfp_hip_index_import(const void* stream_begin, const uint64_t* offsets, size_t count);
 void zfp_hip_index_clear(void);
 
+/*
+ * Batched codec: no counterpart in the reference, which compresses one field
+ * per zfp_compress() call.  Many device-resident fields of mixed type
+ * (int32, int64, float, double), dimensionality (1-3), size and mode
+ * (fixed rate, precision, accuracy, expert) go through one persistent
+ * kernel launch per pass.  The passes are recorded once into a HIP graph
+ * and replayed by each run.  On the first run, the launch shape
+ * (workgroup size, waves per CU, blocks per task) is chosen by timing a few
+ * candidates; the choice is cached per batch shape.  Each job's stream is
+ * the one zfp_hip_encode would write for it, bit for bit.
+ */
+typedef struct {
+  int type;              /* zfp_type of the field */
+  size_t n[4];           /* extents, 0 for unused dimensions (1-3 dims) */
+  ptrdiff_t s[4];        /* strides in elements, 0 for the default */
+  void* data;            /* device pointer to element 0 */
+  uint minbits, maxbits, maxprec;
+  int minexp;            /* codec parameters, as zfp_stream_set_params */
+  uint64_t* stream;      /* device stream words */
+  uint64_t offset;       /* bit offset of the field's first block */
+  uint64_t nwords;       /* stream capacity in words (decompression bound) */
+  uint64_t* index;       /* variable-rate jobs: device array of nblocks + 1
+                            block offsets in bits relative to `offset`;
+                            compression writes it when non-NULL,
+                            decompression requires it */
+} zfp_hip_job;
+
+typedef struct zfp_hip_batch zfp_hip_batch;
+
+#define ZFP_HIP_BATCH_COMPRESS   0
+#define ZFP_HIP_BATCH_DECOMPRESS 1
+
+/* Validates the jobs and builds the batch; NULL on failure (see
+   zfp_hip_last_error).  The job array is copied; the buffers it points to
+   must stay valid while the batch is used.  hip_stream: a hipStream_t
+   (NULL: the default stream). */
+zfp_hip_batch* zfp_hip_batch_create(const zfp_hip_job* jobs, size_t njobs, int direction, void* hip_stream);
+/* Runs the batch and waits for it; bits[j] (when bits is non-NULL)
+   receives the bits written (compression) or consumed (decompression) by
+   job j.  Returns ZFP_HIP_OK or an error code. */
+int zfp_hip_batch_run(zfp_hip_batch* batch, uint64_t* bits);
+/* The launch shape in use (after the first run). */
+int zfp_hip_batch_shape(const zfp_hip_batch* batch, unsigned* wg_size, unsigned* waves_per_cu,
+                        unsigned* blocks_per_task);
+/* Fixes the launch shape instead of tuning it (0 keeps tuning). */
+int zfp_hip_batch_set_shape(zfp_hip_batch* batch, unsigned wg_size, unsigned waves_per_cu,
+                            unsigned blocks_per_task);
+void zfp_hip_batch_destroy(zfp_hip_batch* batch);
+
 /* Library and device identification; 0 when a gfx950 device is usable. */
 int zfp_hip_device_check(void);
 const char* zfp_hip_version(void);

@@ -133,6 +133,17 @@ _API = {
 _SZ4 = ctypes.c_size_t * 4
 _PD4 = ctypes.c_ssize_t * 4
 
+
+class zfp_hip_job(ctypes.Structure):
+    """One field of a batch (include/zfp_hip.h)."""
+    _fields_ = [("type", ctypes.c_int), ("n", _SZ4), ("s", _PD4), ("data", ctypes.c_void_p),
+                ("minbits", ctypes.c_uint), ("maxbits", ctypes.c_uint), ("maxprec", ctypes.c_uint),
+                ("minexp", ctypes.c_int), ("stream", ctypes.c_void_p), ("offset", ctypes.c_uint64),
+                ("nwords", ctypes.c_uint64), ("index", ctypes.c_void_p)]
+
+
+ZFP_HIP_BATCH_COMPRESS, ZFP_HIP_BATCH_DECOMPRESS = 0, 1
+
 _HIP = {
     "zfp_hip_compress": (ctypes.c_size_t, [zs_p, zf_p]),
     "zfp_hip_decompress": (ctypes.c_size_t, [zs_p, zf_p]),
@@ -149,6 +160,12 @@ _HIP = {
     "zfp_hip_device_check": (ctypes.c_int, []),
     "zfp_hip_version": (ctypes.c_char_p, []),
     "zfp_hip_last_error": (ctypes.c_char_p, []),
+    "zfp_hip_batch_create": (ctypes.c_void_p, [_P(zfp_hip_job), ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]),
+    "zfp_hip_batch_run": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64)]),
+    "zfp_hip_batch_shape": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint), _P(ctypes.c_uint),
+                                           _P(ctypes.c_uint)]),
+    "zfp_hip_batch_set_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint]),
+    "zfp_hip_batch_destroy": (None, [ctypes.c_void_p]),
 }
 
 _libs = {}
@@ -226,3 +243,101 @@ def decode(ztype, n, s, d_data, params, d_words, offset=0, stream=None):
     if rc:
         raise RuntimeError(f"zfp_hip_decode failed with status {rc}")
     return bits.value
+
+
+def stream_params(mode, value, ztype, dims):
+    """(minbits, maxbits, maxprec, minexp) that zfp_stream_set_rate /
+    _set_precision / _set_accuracy give for `mode` in ("rate", "precision",
+    "accuracy") -- computed by the library's own zfp.h API."""
+    api = api_lib()
+    zs = api.zfp_stream_open(None)
+    try:
+        if mode == "rate":
+            api.zfp_stream_set_rate(zs, float(value), ztype, dims, 0)
+        elif mode == "precision":
+            api.zfp_stream_set_precision(zs, int(value))
+        elif mode == "accuracy":
+            api.zfp_stream_set_accuracy(zs, float(value))
+        else:
+            raise ValueError(f"unknown mode {mode!r}")
+        z = zs.contents
+        return (z.minbits, z.maxbits, z.maxprec, z.minexp)
+    finally:
+        api.zfp_stream_close(zs)
+
+
+def max_stream_words(ztype, n, params):
+    """Stream capacity in 64-bit words for a field of extents n
+    (zfp_stream_maximum_size of the library's zfp.h API)."""
+    api = api_lib()
+    zs = api.zfp_stream_open(None)
+    nn = list(n) + [0] * (4 - len(n))
+    f = api.zfp_field_alloc()
+    try:
+        fl = f.contents
+        fl.type = ztype
+        fl.nx, fl.ny, fl.nz, fl.nw = nn
+        api.zfp_stream_set_params(zs, *params)
+        return (api.zfp_stream_maximum_size(zs, f) + 7) // 8
+    finally:
+        api.zfp_field_free(f)
+        api.zfp_stream_close(zs)
+
+
+def make_job(ztype, n, s, d_data, params, d_words, offset=0, nwords=0, d_index=None):
+    """A zfp_hip_job for a device field and stream (params: minbits, maxbits,
+    maxprec, minexp)."""
+    nn, ss = _args4(n, s)
+    j = zfp_hip_job()
+    j.type = ztype
+    j.n = nn
+    j.s = ss
+    j.data = d_data
+    j.minbits, j.maxbits, j.maxprec, j.minexp = params
+    j.stream = d_words
+    j.offset = offset
+    j.nwords = nwords
+    j.index = d_index or None
+    return j
+
+
+class Batch:
+    """A zfp_hip_batch: create once, run many times (graph replays)."""
+
+    def __init__(self, jobs, direction=ZFP_HIP_BATCH_COMPRESS, stream=None):
+        lib = hip_lib()
+        self._n = len(jobs)
+        self._jobs = (zfp_hip_job * self._n)(*jobs)
+        self._h = lib.zfp_hip_batch_create(self._jobs, self._n, direction, ctypes.c_void_p(stream or 0))
+        if not self._h:
+            raise RuntimeError("zfp_hip_batch_create failed: " + lib.zfp_hip_last_error().decode())
+
+    def run(self):
+        """Runs the batch; returns the bits of every job."""
+        bits = (ctypes.c_uint64 * self._n)()
+        rc = hip_lib().zfp_hip_batch_run(self._h, bits)
+        if rc:
+            raise RuntimeError(f"zfp_hip_batch_run failed with status {rc}: "
+                               + hip_lib().zfp_hip_last_error().decode())
+        return list(bits)
+
+    def shape(self):
+        a, b, c = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_uint()
+        hip_lib().zfp_hip_batch_shape(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def set_shape(self, wg, waves_per_cu, blocks_per_task):
+        rc = hip_lib().zfp_hip_batch_set_shape(self._h, wg, waves_per_cu, blocks_per_task)
+        if rc:
+            raise RuntimeError("zfp_hip_batch_set_shape failed: " + hip_lib().zfp_hip_last_error().decode())
+
+    def close(self):
+        if self._h:
+            hip_lib().zfp_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

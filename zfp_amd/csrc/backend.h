@@ -7,6 +7,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "codec_block.h"
 #include "zfp_hip.h"
 
@@ -29,10 +31,23 @@ int record_error(int code, const char* what);
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return record_hip_error(e_, #x); } while (0)
 
+// the library lock: every entry point that touches shared state holds it
+extern std::mutex g_mutex;
+// field descriptor from zfp-style extents and strides (0 = default)
+int make_desc(int type, const size_t* n, const ptrdiff_t* s, Desc& d);
+// 0 when the current device is a usable gfx950 (lock held)
+int device_check();
+// whether p is device (or managed) memory
+bool is_device_ptr(const void* p);
+
 // zero words [w0, w1) of a stream, keeping the low `keep` bits of word w0
 __global__ void k_prepare(u64* out, u64 w0, u64 w1, u32 keep);
 // exclusive prefix sum of n u64 values (in and out may not alias)
 int exclusive_scan(const u64* in, u64* out, u64 n, hipStream_t st);
+// the same with a caller-owned temporary of scan_tmp_words(n) words; it
+// allocates nothing, so it may be captured into a HIP graph
+u64 scan_tmp_words(u64 n);
+int exclusive_scan_tmp(const u64* in, u64* out, u64 n, u64* tmp, hipStream_t st);
 
 // grow-only device buffer
 struct Scratch {

@@ -1,0 +1,7 @@
+// batch_dec.hip -- instantiation of the batched codec's DEC pass.
+#include "batch_kernels.h"
+
+namespace zfphip {
+template int launch_batch_pass<BK_DEC>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32, u32, u64*,
+                                   const u64*);
+} // namespace zfphip

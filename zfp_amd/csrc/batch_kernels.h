@@ -1,0 +1,140 @@
+// batch_kernels.h -- device side of the batched codec: many fields of mixed
+// type (int32, int64, float, double), dimensionality (1-3), size and mode go
+// through one persistent launch per pass.
+//
+// Work is a list of tasks (job, first lane); each wave takes tasks in turn
+// (task = wave id, + number of waves, ...) and runs up to blocks_per_task
+// blocks of that job, 64 at a time, one block per lane, with the same
+// per-block codec as the single-field kernels.  The job's kind selects the
+// codec instantiation; it is uniform across the wave, so the switch costs a
+// scalar branch.
+//
+// Passes: LEN (block lengths of variable-rate jobs), then an exclusive scan
+// of those lengths, ENC (every job writes its blocks: fixed rate at
+// b * maxbits, variable rate at its scanned offset) and, for decompression,
+// DEC.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "backend.h"
+#include "codec_block.h"
+#include "codec_device.h"
+
+namespace zfphip {
+
+enum { BK_LEN = 0, BK_ENC = 1, BK_DEC = 2 };
+enum { BM_FIXED = 0, BM_VAR = 1 };
+
+// one field of a batch, as the device sees it
+struct BJob {
+  Geom g;
+  Params z;
+  void* data;    // field, element 0
+  u64* words;    // stream words
+  u64 offset;    // bit offset of the field's first block in `words`
+  u64* index;    // variable rate: nblocks + 1 block offsets relative to `offset`
+  u64 vbase;     // slot of the first block in the batch's length/offset arrays
+  u64 nwords;    // readable words of `words` (decompression bound)
+  u32 nblocks;
+  u32 kind;      // 3 * (zfp_type - 1) + (dims - 1)
+  u32 mode;      // BM_FIXED or BM_VAR
+  u32 pad;
+};
+
+template <typename S, int D, int PASS>
+__device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const u64* offs)
+{
+  u32 bc[4] = {0, 0, 0, 0};
+  const u32 b = lane_block<D>(t, J.g, bc);
+  if constexpr (PASS == BK_LEN) {
+    BitCounter c;
+    c.pos = 0;
+    lens[J.vbase + b] = encode_block<S, D>(c, (const S*)J.data, J.g, bc, J.z);
+  }
+  else if constexpr (PASS == BK_ENC) {
+    u64 s0, s1;
+    if (J.mode == BM_FIXED) {
+      s0 = (u64)b * J.z.maxbits;
+      s1 = s0 + J.z.maxbits;
+    }
+    else {
+      const u64 base = offs[J.vbase];
+      s0 = offs[J.vbase + b] - base;
+      s1 = offs[J.vbase + b + 1] - base;
+      if (J.index)
+        J.index[b] = s0;
+    }
+    BitWriter<false> w;
+    w.begin(J.words, J.offset + s0, J.offset + s1);
+    encode_block<S, D>(w, (const S*)J.data, J.g, bc, J.z);
+    w.finish();
+  }
+  else {
+    const u64 s0 = J.mode == BM_FIXED ? (u64)b * J.z.maxbits : J.index[b];
+    BitReader r;
+    r.begin(J.words, J.nwords, J.offset + s0);
+    decode_block<S, D, true>(r, (S*)J.data, J.g, bc, J.z);
+  }
+}
+
+template <int PASS>
+__device__ __forceinline__ void batch_dispatch(const BJob& J, u32 t, u64* lens, const u64* offs)
+{
+  switch (J.kind) {
+    case 0: batch_one<i32, 1, PASS>(J, t, lens, offs); break;
+    case 1: batch_one<i32, 2, PASS>(J, t, lens, offs); break;
+    case 2: batch_one<i32, 3, PASS>(J, t, lens, offs); break;
+    case 3: batch_one<i64, 1, PASS>(J, t, lens, offs); break;
+    case 4: batch_one<i64, 2, PASS>(J, t, lens, offs); break;
+    case 5: batch_one<i64, 3, PASS>(J, t, lens, offs); break;
+    case 6: batch_one<float, 1, PASS>(J, t, lens, offs); break;
+    case 7: batch_one<float, 2, PASS>(J, t, lens, offs); break;
+    case 8: batch_one<float, 3, PASS>(J, t, lens, offs); break;
+    case 9: batch_one<double, 1, PASS>(J, t, lens, offs); break;
+    case 10: batch_one<double, 2, PASS>(J, t, lens, offs); break;
+    default: batch_one<double, 3, PASS>(J, t, lens, offs); break;
+  }
+}
+
+// persistent kernel of one pass; tasks[i] = {job, first lane}
+template <int PASS>
+__global__ void __launch_bounds__(256, 2)
+k_batch(const BJob* __restrict__ jobs, const uint2* __restrict__ tasks, u32 ntasks, u32 per_task,
+        u64* __restrict__ lens, const u64* __restrict__ offs)
+{
+  const u32 lane = threadIdx.x & 63;
+  const u32 wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const u32 nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (u32 task = wave; task < ntasks; task += nwaves) {
+    const uint2 tk = tasks[task];
+    const BJob& J = jobs[tk.x];
+    const u32 end = umin(tk.y + per_task, J.nblocks);
+    for (u32 t0 = tk.y; t0 < end; t0 += 64) {
+      const u32 t = t0 + lane;
+      if (t < end)
+        batch_dispatch<PASS>(J, t, lens, offs);
+    }
+  }
+}
+
+// host side: launch one pass.  Each pass is instantiated in its own
+// translation unit (batch_len.hip, batch_enc.hip, batch_dec.hip) so that the
+// three compile in parallel.
+template <int PASS>
+int launch_batch_pass(unsigned grid, unsigned wg, hipStream_t st, const BJob* jobs, const uint2* tasks,
+                      u32 ntasks, u32 per_task, u64* lens, const u64* offs)
+{
+  hipLaunchKernelGGL((k_batch<PASS>), dim3(grid), dim3(wg), 0, st, jobs, tasks, ntasks, per_task, lens, offs);
+  HIPCHK(hipGetLastError());
+  return ZFP_HIP_OK;
+}
+
+extern template int launch_batch_pass<BK_LEN>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32,
+                                              u32, u64*, const u64*);
+extern template int launch_batch_pass<BK_ENC>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32,
+                                              u32, u64*, const u64*);
+extern template int launch_batch_pass<BK_DEC>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32,
+                                              u32, u64*, const u64*);
+
+} // namespace zfphip

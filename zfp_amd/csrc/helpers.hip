@@ -93,14 +93,20 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_add(u64* out, u64 n, const u64
 }
 
 
+u64 scan_tmp_words(u64 n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
 int exclusive_scan(const u64* in, u64* out, u64 n, hipStream_t st)
 {
-  u64 ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   void* tp;
-  int rc = g_scan_tmp.get(ntiles * sizeof(u64), &tp);
+  int rc = g_scan_tmp.get(scan_tmp_words(n) * sizeof(u64), &tp);
   if (rc)
     return rc;
-  u64* sums = (u64*)tp;
+  return exclusive_scan_tmp(in, out, n, (u64*)tp, st);
+}
+
+int exclusive_scan_tmp(const u64* in, u64* out, u64 n, u64* sums, hipStream_t st)
+{
+  const u64 ntiles = scan_tmp_words(n);
   hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_BS), 0, st, in, out, n, sums);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BS), 0, st, sums, ntiles);
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)ntiles), dim3(SCAN_BS), 0, st, out, n, (const u64*)sums);

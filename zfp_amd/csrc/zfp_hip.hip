@@ -35,10 +35,11 @@ int record_hip_error(hipError_t e, const char* what)
   return ZFP_HIP_ERUNTIME;
 }
 
-namespace {
-
-Scratch g_field, g_stream;
 std::mutex g_mutex;
+
+namespace {
+Scratch g_field, g_stream;
+} // namespace
 
 int make_desc(int type, const size_t* n, const ptrdiff_t* s, Desc& d)
 {
@@ -118,6 +119,8 @@ int dispatch_decode(const Desc& d, void* data, const Params& z, const u64* in, u
   return ZFP_HIP_EINVAL;
 }
 
+namespace {
+
 // Block index of a variable-rate stream: offsets[b] relative to `start`
 // (nblocks + 1 entries, on the device).  Keyed by the stream buffer's begin
 // pointer; an entry matches a decode only at the same start offset, block
@@ -164,6 +167,8 @@ const Index* index_find(const void* key, u64 start, const Desc& d, const size_t*
 
 int g_dev_state = -1; // -1 unknown, 0 usable, else an error code
 
+} // namespace
+
 // called with the library lock held; also clears HIP error state left by
 // other users of the runtime in this thread, so that the checks after our
 // own calls see only our errors
@@ -207,6 +212,8 @@ bool is_device_ptr(const void* p)
   }
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
+
+namespace {
 
 // lowest and highest element offsets of a field from element 0 (src/zfp.c:18-39)
 void field_span(const Desc& d, i64* lo, i64* hi)
