@@ -43,6 +43,9 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the batched-codec measurement")
     p.add_argument("--batch-arrays", type=int, default=1024)
     p.add_argument("--batch-reps", type=int, default=9)
+    p.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    p.add_argument("--serve-fields", type=int, default=32)
+    p.add_argument("--serve-requests", type=int, default=200)
     return p.parse_args()
 
 
@@ -302,6 +305,129 @@ def batch_bench(args):
     }
 
 
+def serve_bench(args):
+    """Serving loop: a pool of resident 3D float32/float64 fields (volume
+    log-uniform 32^3..384^3; fixed rate U{4..32} or accuracy 10^U(-6,-2)),
+    and R requests (50/50 compress/decompress of a random pool field) that
+    arrive as a Poisson process at 0.3/0.7/0.95 of the capacity measured
+    with all requests queued at once.  Latency is submission to completion,
+    per request; every result is compared with the same job run alone."""
+    import numpy as np
+    import torch
+    import zfp_amd as Z
+
+    rng = np.random.default_rng(20251204)
+    pool = []
+    for k in range(args.serve_fields):
+        e = int(round(math.exp(rng.uniform(math.log(32), math.log(384)))))
+        n = [e, e, e]
+        ztype = 4 if k % 2 == 0 else 3
+        if rng.random() < 0.5:
+            p = Z.stream_params("rate", int(rng.integers(4, 33)), ztype, 3)
+        else:
+            p = Z.stream_params("accuracy", float(10.0 ** rng.uniform(-6, -2)), ztype, 3)
+        c = torch.arange(e, device="cuda:0", dtype=torch.float64) * (2 * math.pi / e)
+        f = (torch.sin(3 * c).view(1, 1, e) * torch.cos(2 * c).view(1, e, 1) + torch.sin(c + 1).view(e, 1, 1)
+             + 1e-3 * torch.randn((e, e, e), device="cuda:0", dtype=torch.float64))
+        f = f if ztype == 4 else f.float()
+        nw = Z.max_stream_words(ztype, n, p)
+        nb = (e // 4 + (e % 4 > 0)) ** 3
+        var = p[0] != p[1]
+        # reference results: the job alone, one call each
+        w = torch.zeros(nw, dtype=torch.int64, device="cuda:0")
+        bits = Z.encode(ztype, n, None, f.data_ptr(), p, w.data_ptr())
+        o = torch.empty_like(f)
+        Z.decode(ztype, n, None, o.data_ptr(), p, w.data_ptr())
+        idx = None
+        if var:
+            # the index of the stream, from a one-job batch (same kernels)
+            idx = torch.zeros(nb + 1, dtype=torch.int64, device="cuda:0")
+            wi = torch.zeros_like(w)
+            b1 = Z.Batch([Z.make_job(ztype, n, None, f.data_ptr(), p, wi.data_ptr(), 0, nw, idx.data_ptr())])
+            b1.run()
+            b1.close()
+            del wi
+        pool.append(dict(ztype=ztype, n=n, p=p, f=f, nw=nw, nb=nb, var=var, w=w, bits=bits, o=o, idx=idx,
+                         ubytes=f.numel() * f.element_size()))
+    torch.cuda.synchronize()
+
+    R = args.serve_requests
+    picks = [(int(rng.integers(0, len(pool))), "c" if rng.random() < 0.5 else "d") for _ in range(R)]
+    # per-request output buffers, allocated once
+    outs = []
+    for fi, kind in picks:
+        P = pool[fi]
+        if kind == "c":
+            outs.append((torch.zeros(P["nw"], dtype=torch.int64, device="cuda:0"),
+                         torch.zeros(P["nb"] + 1, dtype=torch.int64, device="cuda:0") if P["var"] else None))
+        else:
+            outs.append((torch.empty_like(P["f"]), None))
+    torch.cuda.synchronize()
+
+    def job(i):
+        fi, kind = picks[i]
+        P = pool[fi]
+        buf, idx = outs[i]
+        if kind == "c":
+            return (Z.make_job(P["ztype"], P["n"], None, P["f"].data_ptr(), P["p"], buf.data_ptr(), 0, P["nw"],
+                               idx.data_ptr() if idx is not None else None), Z.ZFP_HIP_BATCH_COMPRESS)
+        return (Z.make_job(P["ztype"], P["n"], None, buf.data_ptr(), P["p"], P["w"].data_ptr(), 0, P["nw"],
+                           P["idx"].data_ptr() if P["var"] else None), Z.ZFP_HIP_BATCH_DECOMPRESS)
+
+    jobs = [job(i) for i in range(R)]
+    nbytes = [pool[fi]["ubytes"] + pool[fi]["nw"] * 8 for fi, _ in picks]
+    ubytes = sum(pool[fi]["ubytes"] for fi, _ in picks)
+
+    def run(arrivals):
+        sv = Z.Server(64 << 30, 0, 2)
+        t0 = time.perf_counter()
+        ids = []
+        for i in range(R):
+            while arrivals is not None and time.perf_counter() - t0 < arrivals[i]:
+                pass
+            ids.append(sv.submit(jobs[i][0], jobs[i][1], nbytes[i]))
+        res = [sv.wait(r) for r in ids]
+        t1 = time.perf_counter()
+        st = sv.stats()
+        sv.close()
+        return res, t1 - t0, st
+
+    run(None)   # warm-up
+    res, t_all, _ = run(None)
+    capacity = ubytes / t_all   # bytes/s with everything queued at once
+    mean_req = ubytes / R
+    loads = {}
+    exact = True
+    for L in (0.3, 0.7, 0.95):
+        lam = L * capacity / mean_req
+        arr = np.cumsum(rng.exponential(1.0 / lam, size=R))
+        res, t_run, st = run(arr)
+        lat = sorted(x[1] for x in res)
+        loads[str(L)] = {"offered_req_per_s": round(lam, 1), "p50_ms": round(lat[len(lat) // 2], 3),
+                         "p99_ms": round(lat[min(R - 1, int(0.99 * (R - 1) + 0.5))], 3),
+                         "GBps": round(ubytes / t_run / 1e9, 1), "batches": st["batches"],
+                         "max_batch": st["max_batch"], "max_inflight": st["max_inflight"]}
+        # every result equals the job run alone
+        for i, (fi, kind) in enumerate(picks):
+            P = pool[fi]
+            bits = res[i][0]
+            if bits != P["bits"]:
+                exact = False
+            elif kind == "c":
+                nw = (bits + 63) // 64
+                exact &= bool(torch.equal(outs[i][0][:nw], P["w"][:nw]))
+            else:
+                exact &= bool(torch.equal(outs[i][0].view(torch.uint8), P["o"].view(torch.uint8)))
+    return {
+        "workload": f"{R} requests (50/50 compress/decompress) over {len(pool)} resident 3D f32/f64 fields, "
+                    "volume log-uniform 32^3..384^3, rate U{4..32} or accuracy 10^U(-6,-2), "
+                    "Poisson arrivals, 64 GiB HBM budget, 2 batches in flight",
+        "capacity_GBps": round(capacity / 1e9, 1),
+        "loads": loads,
+        "bit_exact_vs_alone": exact,
+    }
+
+
 def cpu_baseline(args, f_dev):
     """Reference serial zfp on a 128^3 slab of the benchmark field (or the
     oracle when the reference build is absent)."""
@@ -372,6 +498,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(args, f)
     del f
     batched = None if args.no_batch else batch_bench(args)
+    serving = None if args.no_serve else serve_bench(args)
     line = {
         "metric": "compress/decompress GB/s (uncompressed) and % HBM peak; batched vs per-call",
         "value": round(value, 1),
@@ -393,6 +520,7 @@ def main():
         "passes": detail,
         "parity_checks": checks,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line))
 

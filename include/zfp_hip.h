@@ -140,6 +140,41 @@ int zfp_hip_batch_set_shape(zfp_hip_batch* batch, unsigned wg_size, unsigned wav
                             unsigned blocks_per_task);
 void zfp_hip_batch_destroy(zfp_hip_batch* batch);
 
+/*
+ * Serving loop: no counterpart in the reference.  A scheduler thread admits
+ * submitted compression and decompression requests in arrival order while
+ * their declared device footprint fits the HBM budget, and gathers every
+ * admitted request that is not yet running into one batch.  A batch is one
+ * persistent launch per pass (the kernels of zfp_hip_batch_*).  Up to
+ * `slots` batches run at once on their own HIP streams, so requests that
+ * arrive while others are on the GPU start as soon as a slot and budget
+ * free up.  Fields and streams are the caller's device buffers and stay
+ * resident.  Each request's result is bit-identical to the same job run
+ * alone.
+ */
+typedef struct zfp_hip_server zfp_hip_server;
+
+/* hbm_budget: bytes that admitted requests may declare in total;
+   max_batch_jobs: jobs per batch (0: 1024); slots: concurrent batches
+   (0: 2).  NULL on failure. */
+zfp_hip_server* zfp_hip_server_create(uint64_t hbm_budget, unsigned max_batch_jobs, unsigned slots);
+/* Queues one job.  direction: ZFP_HIP_BATCH_COMPRESS or _DECOMPRESS.
+   bytes: the job's device footprint for admission (field plus stream).
+   hip_stream: the stream that produced the job's inputs; the job starts
+   after the work queued on it so far.  *id receives the request id. */
+int zfp_hip_server_submit(zfp_hip_server* server, const zfp_hip_job* job, int direction, uint64_t bytes,
+                          void* hip_stream, uint64_t* id);
+/* Waits for request `id`; *bits receives its bits written or consumed and
+   *latency_ms the time from submission to completion.  Returns the
+   request's status. */
+int zfp_hip_server_wait(zfp_hip_server* server, uint64_t id, uint64_t* bits, double* latency_ms);
+/* Counters since creation: requests done, batches launched, the largest
+   number of requests in one batch and the most batches in flight at once. */
+int zfp_hip_server_stats(zfp_hip_server* server, uint64_t* done, uint64_t* batches, uint64_t* max_batch,
+                         uint64_t* max_inflight);
+/* Finishes all queued work, then stops the scheduler. */
+void zfp_hip_server_destroy(zfp_hip_server* server);
+
 /* Library and device identification; 0 when a gfx950 device is usable. */
 int zfp_hip_device_check(void);
 const char* zfp_hip_version(void);

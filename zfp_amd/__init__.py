@@ -166,6 +166,14 @@ _HIP = {
                                            _P(ctypes.c_uint)]),
     "zfp_hip_batch_set_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint]),
     "zfp_hip_batch_destroy": (None, [ctypes.c_void_p]),
+    "zfp_hip_server_create": (ctypes.c_void_p, [ctypes.c_uint64, ctypes.c_uint, ctypes.c_uint]),
+    "zfp_hip_server_submit": (ctypes.c_int, [ctypes.c_void_p, _P(zfp_hip_job), ctypes.c_int, ctypes.c_uint64,
+                                             ctypes.c_void_p, _P(ctypes.c_uint64)]),
+    "zfp_hip_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _P(ctypes.c_uint64),
+                                           _P(ctypes.c_double)]),
+    "zfp_hip_server_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), _P(ctypes.c_uint64),
+                                            _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
+    "zfp_hip_server_destroy": (None, [ctypes.c_void_p]),
 }
 
 _libs = {}
@@ -334,6 +342,48 @@ class Batch:
     def close(self):
         if self._h:
             hip_lib().zfp_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """A zfp_hip_server: submit jobs as they arrive, wait for their results."""
+
+    def __init__(self, hbm_budget=64 << 30, max_batch_jobs=0, slots=0):
+        self._h = hip_lib().zfp_hip_server_create(hbm_budget, max_batch_jobs, slots)
+        if not self._h:
+            raise RuntimeError("zfp_hip_server_create failed: " + hip_lib().zfp_hip_last_error().decode())
+
+    def submit(self, job, direction, nbytes, stream=None):
+        rid = ctypes.c_uint64()
+        rc = hip_lib().zfp_hip_server_submit(self._h, ctypes.byref(job), direction, nbytes,
+                                             ctypes.c_void_p(stream or 0), ctypes.byref(rid))
+        if rc:
+            raise RuntimeError(f"zfp_hip_server_submit failed with status {rc}: "
+                               + hip_lib().zfp_hip_last_error().decode())
+        return rid.value
+
+    def wait(self, rid):
+        """(bits, latency in ms) of request rid."""
+        bits, lat = ctypes.c_uint64(), ctypes.c_double()
+        rc = hip_lib().zfp_hip_server_wait(self._h, rid, ctypes.byref(bits), ctypes.byref(lat))
+        if rc:
+            raise RuntimeError(f"request {rid} failed with status {rc}")
+        return bits.value, lat.value
+
+    def stats(self):
+        v = [ctypes.c_uint64() for _ in range(4)]
+        hip_lib().zfp_hip_server_stats(self._h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("done", "batches", "max_batch", "max_inflight"), (x.value for x in v)))
+
+    def close(self):
+        if self._h:
+            hip_lib().zfp_hip_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -36,7 +36,7 @@ __global__ void k_batch_prepare(const BJob* __restrict__ jobs, const u32* __rest
 // Bits written (compression) or consumed (decompression) per job; a
 // variable-rate compression also completes the job's index with its total.
 __global__ void k_batch_totals(const BJob* __restrict__ jobs, u32 njobs, const u64* __restrict__ offs,
-                               u64* __restrict__ bits, int decode)
+                               u64* __restrict__ bits)
 {
   const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= njobs)
@@ -45,7 +45,7 @@ __global__ void k_batch_totals(const BJob* __restrict__ jobs, u32 njobs, const u
   u64 b;
   if (J.mode == BM_FIXED)
     b = (u64)J.nblocks * J.z.maxbits;
-  else if (!decode) {
+  else if (!J.dir) {
     b = offs[J.vbase + J.nblocks] - offs[J.vbase];
     if (J.index)
       J.index[J.nblocks] = b;
@@ -53,6 +53,21 @@ __global__ void k_batch_totals(const BJob* __restrict__ jobs, u32 njobs, const u
   else
     b = J.index[J.nblocks];
   bits[j] = b;
+}
+
+int launch_batch_prepare(unsigned grid, hipStream_t st, const BJob* jobs, const u32* pjobs, u32 npjobs,
+                         const u64* offs)
+{
+  hipLaunchKernelGGL(k_batch_prepare, dim3(grid), dim3(256), 0, st, jobs, pjobs, npjobs, offs);
+  HIPCHK(hipGetLastError());
+  return ZFP_HIP_OK;
+}
+
+int launch_batch_totals(hipStream_t st, const BJob* jobs, u32 njobs, const u64* offs, u64* bits)
+{
+  hipLaunchKernelGGL(k_batch_totals, dim3((njobs + 255) / 256), dim3(256), 0, st, jobs, njobs, offs, bits);
+  HIPCHK(hipGetLastError());
+  return ZFP_HIP_OK;
 }
 
 namespace {
@@ -163,24 +178,16 @@ struct Batch {
         if (!rc)
           rc = exclusive_scan_tmp(d_lens, d_offs, nvar + 1, d_tmp, st);
       }
-      if (!rc && !prep.empty()) {
-        hipLaunchKernelGGL(k_batch_prepare, dim3(ncu * 4), dim3(256), 0, st, (const BJob*)d_jobs,
-                           (const u32*)d_prep, (u32)prep.size(), (const u64*)d_offs);
-        if (hipGetLastError() != hipSuccess)
-          rc = record_error(ZFP_HIP_ERUNTIME, "batch: prepare launch failed");
-      }
+      if (!rc && !prep.empty())
+        rc = launch_batch_prepare(ncu * 4, st, d_jobs, d_prep, (u32)prep.size(), d_offs);
       if (!rc)
         rc = launch_batch_pass<BK_ENC>(grid, s.wg, st, d_jobs, d_tasks, (u32)all.size(), s.bpt, d_lens, d_offs);
     }
     else {
       rc = launch_batch_pass<BK_DEC>(grid, s.wg, st, d_jobs, d_tasks, (u32)all.size(), s.bpt, d_lens, d_offs);
     }
-    if (!rc) {
-      hipLaunchKernelGGL(k_batch_totals, dim3((njobs + 255) / 256), dim3(256), 0, st, (const BJob*)d_jobs, njobs,
-                         (const u64*)d_offs, d_bits, dir == ZFP_HIP_BATCH_DECOMPRESS ? 1 : 0);
-      if (hipGetLastError() != hipSuccess)
-        rc = record_error(ZFP_HIP_ERUNTIME, "batch: totals launch failed");
-    }
+    if (!rc)
+      rc = launch_batch_totals(st, d_jobs, njobs, d_offs, d_bits);
     if (!rc && hipMemcpyAsync(h_bits, d_bits, njobs * sizeof(u64), hipMemcpyDeviceToHost, st) != hipSuccess)
       rc = record_error(ZFP_HIP_ERUNTIME, "batch: capture of the result copy failed");
     hipGraph_t graph = nullptr;
@@ -255,8 +262,6 @@ struct Batch {
   }
 };
 
-namespace {
-
 int job_to_device(const zfp_hip_job& in, int dir, BJob& J, u64& vbase)
 {
   memset(&J, 0, sizeof(J));
@@ -279,7 +284,8 @@ int job_to_device(const zfp_hip_job& in, int dir, BJob& J, u64& vbase)
   J.nblocks = d.nblocks;
   J.kind = 3u * (u32)(in.type - 1) + (u32)(d.dims - 1);
   J.mode = in.minbits == in.maxbits ? BM_FIXED : BM_VAR;
-  if (J.mode == BM_VAR) {
+  J.dir = dir == ZFP_HIP_BATCH_DECOMPRESS ? 1u : 0u;
+  if (J.mode == BM_VAR && dir == ZFP_HIP_BATCH_COMPRESS) {
     J.vbase = vbase;
     vbase += d.nblocks;
   }
@@ -295,6 +301,8 @@ int job_to_device(const zfp_hip_job& in, int dir, BJob& J, u64& vbase)
   }
   return ZFP_HIP_OK;
 }
+
+namespace {
 
 std::string signature(const Batch& b)
 {

@@ -39,7 +39,7 @@ struct BJob {
   u32 nblocks;
   u32 kind;      // 3 * (zfp_type - 1) + (dims - 1)
   u32 mode;      // BM_FIXED or BM_VAR
-  u32 pad;
+  u32 dir;       // 0 compression, 1 decompression
 };
 
 template <typename S, int D, int PASS>
@@ -129,6 +129,18 @@ int launch_batch_pass(unsigned grid, unsigned wg, hipStream_t st, const BJob* jo
   HIPCHK(hipGetLastError());
   return ZFP_HIP_OK;
 }
+
+// checks a job and fills its device descriptor; variable-rate compression
+// jobs take the next nblocks slots of the length/offset arrays from vbase
+int job_to_device(const zfp_hip_job& in, int dir, BJob& J, u64& vbase);
+
+// zero the stream words of the listed jobs that blocks OR into
+// (batch.hip); offs holds the scanned offsets of variable-rate jobs
+int launch_batch_prepare(unsigned grid, hipStream_t st, const BJob* jobs, const u32* pjobs, u32 npjobs,
+                         const u64* offs);
+// bits[j]: bits written (compression jobs) or consumed (decompression
+// jobs) by job j; completes the index of variable-rate compression jobs
+int launch_batch_totals(hipStream_t st, const BJob* jobs, u32 njobs, const u64* offs, u64* bits);
 
 extern template int launch_batch_pass<BK_LEN>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32,
                                               u32, u64*, const u64*);
