@@ -12,11 +12,22 @@ One step = compress + decompress at each of the three rates for both layouts
 throughput of those passes: 12 * 1 GiB / step time.  The roofline entry is
 for the dominant kernel (compression at rate 16, contiguous): algorithmic
 bytes N*8 + C per launch over its mean duration, timed with HIP events on
-the stream it is launched on.  The CPU baseline is the reference's serial
-zfp_compress/zfp_decompress (and its OpenMP compression) on a 128^3 slab of
-the same field, timed on this machine's host cores.
+the stream it is launched on; its `traffic` is the HBM bytes per launch that
+the committed rocprofv3 counter summary (profiles/) measured.  The CPU
+baseline is the reference's serial zfp_compress/zfp_decompress (and its
+OpenMP compression) on a 128^3 slab of the same field, timed on this
+machine's host cores.
+
+Beside the headline line's own workload, the same JSON line reports:
+  batched        1024 mixed arrays in one persistent launch per pass (graph
+                 replay) against one zfp_hip_encode/decode call per array;
+  serving        the request loop at three Poisson offered loads (p50/p99);
+  variable_rate  precision 16/28 and accuracy 1e-3/1e-7 on a 2D float32
+                 8191x8193 and a 4D float64 96x96x96x24 array.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
+       [--rates 4,16,32] [--layouts contiguous,permuted]
+       [--no-cpu] [--no-batch] [--no-serve] [--no-vr]
 """
 import argparse
 import ctypes
@@ -43,7 +54,13 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the batched-codec measurement")
     p.add_argument("--batch-arrays", type=int, default=1024)
     p.add_argument("--batch-reps", type=int, default=9)
+    p.add_argument("--rates", default="4,16,32", help="fixed rates of the 512^3 passes")
+    p.add_argument("--layouts", default="contiguous,permuted", help="layouts of the 512^3 passes")
+    p.add_argument("--no-vr", action="store_true", help="skip the variable-rate measurement")
     p.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only copy 1 GiB to 1 GiB three times (the known-size kernel that scales the "
+                        "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     p.add_argument("--serve-fields", type=int, default=32)
     p.add_argument("--serve-requests", type=int, default=200)
     return p.parse_args()
@@ -74,6 +91,18 @@ def make_field(torch, n, seed=20251202, special=0.01):
     return f
 
 
+def measured_traffic(dom):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by what the
+    same counters read for a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "k_encode_f64_3d_512_counters.json")
+    if dom != ("compress", 16, "contiguous") or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        c = json.load(fh)
+    return int(c["fetch_bytes_scaled"] + c["write_bytes_scaled"])
+
+
 def gpu_bench(args):
     import torch
     import zfp_amd as Z
@@ -90,8 +119,9 @@ def gpu_bench(args):
         "contiguous": (f.data_ptr(), [0, 0, 0, 0]),
         "permuted": (fp.data_ptr() + (n - 1) * n * n * 8, [-n * n, n, 1, 0]),
     }
+    layouts = {k: v for k, v in layouts.items() if k in args.layouts.split(",")}
     out = torch.empty_like(f)
-    rates = [4, 16, 32]
+    rates = [int(r) for r in args.rates.split(",")]
     streams = {r: torch.zeros(N * r // 64 + 8, dtype=torch.int64, device="cuda:0") for r in rates}
     st = torch.cuda.Stream()  # default priority
     sh = st.cuda_stream
@@ -112,22 +142,24 @@ def gpu_bench(args):
     # fixed-rate size law holds, decompression consumes what compression
     # wrote, and both layouts decompress to the same values
     checks = {}
+    lay0 = next(iter(layouts))
     with torch.cuda.stream(st):  # torch's copies and the codec run in order on one stream
         for r in rates:
-            bits_c = run_pass("compress", r, "contiguous")
-            s1 = streams[r].clone()
-            bits_p = run_pass("compress", r, "permuted")
-            same = bool(torch.equal(s1, streams[r]))
-            del s1
-            bits_d = run_pass("decompress", r, "contiguous")
-            o1 = out.clone()
-            run_pass("decompress", r, "permuted")
-            same_out = bool(torch.equal(o1.view(torch.int64), torch.flip(out.permute(2, 1, 0), dims=[2])
-                                        .contiguous().view(torch.int64)))
-            del o1
-            checks[f"rate{r}"] = {"bits": bits_c, "size_law": bits_c == N * r,
-                                  "layouts_equal": same and bits_p == bits_c, "decode_bits": bits_d == bits_c,
-                                  "layouts_decode_equal": same_out}
+            bits_c = run_pass("compress", r, lay0)
+            bits_d = run_pass("decompress", r, lay0)
+            checks[f"rate{r}"] = {"bits": bits_c, "size_law": bits_c == N * r, "decode_bits": bits_d == bits_c}
+            if len(layouts) == 2:
+                s1 = streams[r].clone()
+                bits_p = run_pass("compress", r, "permuted")
+                same = bool(torch.equal(s1, streams[r]))
+                del s1
+                o1 = out.clone()
+                run_pass("decompress", r, "permuted")
+                same_out = bool(torch.equal(o1.view(torch.int64), torch.flip(out.permute(2, 1, 0), dims=[2])
+                                            .contiguous().view(torch.int64)))
+                del o1
+                checks[f"rate{r}"].update({"layouts_equal": same and bits_p == bits_c,
+                                           "layouts_decode_equal": same_out})
     torch.cuda.synchronize()
 
     for _ in range(args.warmup):
@@ -164,12 +196,15 @@ def gpu_bench(args):
             "hbm_frac": round((ubytes + cbytes) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         }
     dom = ("compress", 16, "contiguous")
+    if dom not in per:
+        dom = passes[0]
     dom_ms = per[dom]
-    traffic_bytes = ubytes + N * 16 // 8
+    traffic_bytes = ubytes + N * dom[1] // 8
     achieved = traffic_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_encode<double,3,fixed> rate 16 contiguous",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+                "kernel": f"k_encode<double,3,fixed> rate {dom[1]} {dom[2]}" if dom[0] == "compress"
+                          else f"k_decode<double,3> rate {dom[1]} {dom[2]}",
                 "algorithmic_bytes_per_launch": traffic_bytes, "mean_launch_ms": round(dom_ms, 4)}
     return value, ms_step, roofline, detail, checks, f
 
@@ -303,6 +338,79 @@ def batch_bench(args):
         "shape": {"wg": shape[0], "waves_per_cu": shape[1], "blocks_per_task": shape[2]},
         "bit_exact_vs_per_call": {"streams": same_streams, "arrays": same_arrays},
     }
+
+
+def vr_fields(torch):
+    """2D float32 8191x8193 and 4D float64 96x96x96x24: smooth seeded fields
+    with a 10% box of U(-1,1) noise (x fastest in memory)."""
+    g = torch.Generator(device="cuda:0").manual_seed(20251205)
+    out = {}
+    ny, nx = 8193, 8191
+    cx = torch.arange(nx, device="cuda:0", dtype=torch.float64) * (2 * math.pi / nx)
+    cy = torch.arange(ny, device="cuda:0", dtype=torch.float64) * (2 * math.pi / ny)
+    f = torch.sin(3 * cx).view(1, nx) * torch.cos(2 * cy).view(ny, 1) + 0.5 * torch.sin(5 * cx + 4 * cy.view(ny, 1))
+    by, bx = int(ny * 0.316), int(nx * 0.316)   # ~10% of the area
+    f[:by, :bx] = 2 * torch.rand((by, bx), device="cuda:0", dtype=torch.float64, generator=g) - 1
+    out["2d_f32_8191x8193"] = (3, [nx, ny], f.float().contiguous())
+    n4 = [96, 96, 96, 24]
+    c = [torch.arange(v, device="cuda:0", dtype=torch.float64) * (2 * math.pi / v) for v in n4]
+    f = (torch.sin(2 * c[0]).view(1, 1, 1, 96) * torch.cos(c[1]).view(1, 1, 96, 1)
+         + torch.sin(3 * c[2]).view(1, 96, 1, 1) * torch.cos(2 * c[3]).view(24, 1, 1, 1))
+    f[:, :, :30, :30] = 2 * torch.rand((24, 96, 30, 30), device="cuda:0", dtype=torch.float64, generator=g) - 1
+    out["4d_f64_96x96x96x24"] = (4, n4, f.contiguous())
+    return out
+
+
+def vr_bench(args):
+    """Variable-rate modes (precision 16/28, accuracy 1e-3/1e-7) on the 2D
+    float32 and 4D float64 arrays: length pass, device scan and write pass
+    per compression; decompression from the recorded block index.  Timed
+    per call (wall clock, after a warm-up); checks: decompression consumes
+    exactly the bits compression wrote, and the batched kernels (a one-job
+    batch, 2D only) write the same stream as the per-field path."""
+    import torch
+    import zfp_amd as Z
+
+    res = {}
+    modes = [("precision", 16), ("precision", 28), ("accuracy", 1e-3), ("accuracy", 1e-7)]
+    for name, (ztype, n, f) in vr_fields(torch).items():
+        dims = len(n)
+        ubytes = f.numel() * f.element_size()
+        for mode, val in modes:
+            p = Z.stream_params(mode, val, ztype, dims)
+            nw = Z.max_stream_words(ztype, n, p)
+            w = torch.zeros(nw, dtype=torch.int64, device="cuda:0")
+            o = torch.empty_like(f)
+            bits = Z.encode(ztype, n, None, f.data_ptr(), p, w.data_ptr())
+            dbits = Z.decode(ztype, n, None, o.data_ptr(), p, w.data_ptr())
+            tc, td = [], []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                Z.encode(ztype, n, None, f.data_ptr(), p, w.data_ptr())
+                t1 = time.perf_counter()
+                Z.decode(ztype, n, None, o.data_ptr(), p, w.data_ptr())
+                t2 = time.perf_counter()
+                tc.append(t1 - t0)
+                td.append(t2 - t1)
+            entry = {"bits_per_value": round(bits / f.numel(), 3),
+                     "compress_GBps": round(ubytes / min(tc) / 1e9, 1),
+                     "decompress_GBps": round(ubytes / min(td) / 1e9, 1),
+                     "decode_bits_equal": dbits == bits}
+            if dims <= 3:
+                nb = math.prod((v + 3) // 4 for v in n)
+                wb = torch.zeros_like(w)
+                idx = torch.zeros(nb + 1, dtype=torch.int64, device="cuda:0")
+                b = Z.Batch([Z.make_job(ztype, n, None, f.data_ptr(), p, wb.data_ptr(), 0, nw, idx.data_ptr())])
+                bb = b.run()[0]
+                b.close()
+                k = (bits + 63) // 64
+                entry["batch_stream_equal"] = bb == bits and bool(torch.equal(wb[:k], w[:k]))
+                del wb, idx
+            res[f"{name}_{mode}{val}"] = entry
+            del w, o
+        del f
+    torch.cuda.synchronize()
+    return res
 
 
 def serve_bench(args):
@@ -488,8 +596,22 @@ def cpu_baseline(args, f_dev):
     return out
 
 
+def copy_calibration():
+    import torch
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")   # 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8,
+                                           "launches": 3}}))
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
@@ -499,6 +621,7 @@ def main():
     del f
     batched = None if args.no_batch else batch_bench(args)
     serving = None if args.no_serve else serve_bench(args)
+    variable_rate = None if args.no_vr else vr_bench(args)
     line = {
         "metric": "compress/decompress GB/s (uncompressed) and % HBM peak; batched vs per-call",
         "value": round(value, 1),
@@ -521,6 +644,7 @@ def main():
         "parity_checks": checks,
         "batched": batched,
         "serving": serving,
+        "variable_rate": variable_rate,
     }
     print(json.dumps(line))
 
