@@ -36,9 +36,10 @@ extern "C" {
  * Compresses `field` into the bit stream of `stream` starting at its current
  * write position, including bits still buffered in the stream state (e.g. a
  * header written by zfp_write_header).  Field data and stream buffer may be
- * host or device memory (detected per pointer).  All modes except reversible
- * are supported for 1D-4D int32, int64, float and double fields with any
- * strides.  On success the stream is left flushed exactly as serial
+ * host or device memory (detected per pointer).  All modes (fixed rate,
+ * precision, accuracy, expert and reversible) are supported for 1D-4D
+ * int32, int64, float and double fields with any strides.  On success the
+ * stream is left flushed exactly as serial
  * zfp_compress() leaves it and the return value is the stream size in bytes;
  * on a request the policy declines, it returns 0 and leaves the stream
  * untouched.

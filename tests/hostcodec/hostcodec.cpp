@@ -49,14 +49,18 @@ uint64_t enc(const void* data, const size_t* n, const ptrdiff_t* s, const Params
     block_coords<D>(b, g, bc);
     BitCounter c;
     c.pos = 0;
-    off[b + 1] = off[b] + encode_block<S, D>(c, (const S*)data, g, bc, z);
+    off[b + 1] = off[b] + (z.minexp < -1074 ? encode_block_rev<S, D>(c, (const S*)data, g, bc, z)
+                                            : encode_block<S, D>(c, (const S*)data, g, bc, z));
   }
   for (u32 b = 0; b < nb; b++) {
     u32 bc[4] = {0, 0, 0, 0};
     block_coords<D>(b, g, bc);
     BitWriter<false> w;
     w.begin(words, offset + off[b], offset + off[b + 1]);
-    encode_block<S, D>(w, (const S*)data, g, bc, z);
+    if (z.minexp < -1074)
+      encode_block_rev<S, D>(w, (const S*)data, g, bc, z);
+    else
+      encode_block<S, D>(w, (const S*)data, g, bc, z);
     w.finish();
   }
   return off[nb];
@@ -75,7 +79,8 @@ uint64_t dec(void* data, const size_t* n, const ptrdiff_t* s, const Params& z, c
     block_coords<D>(b, g, bc);
     BitReader r;
     r.begin(words, nwords, pos);
-    pos += decode_block<S, D, true>(r, (S*)data, g, bc, z);
+    pos += z.minexp < -1074 ? decode_block_rev<S, D, true>(r, (S*)data, g, bc, z)
+                            : decode_block<S, D, true>(r, (S*)data, g, bc, z);
   }
   return pos - offset;
 }

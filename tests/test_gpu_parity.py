@@ -44,10 +44,15 @@ def mode_params(name, ztype, dims):
         return stream_params_accuracy(1e-7)
     if name == "expert":
         return (100, 600, 20, -30)
+    if name == "reversible":   # zfp_stream_set_reversible (src/zfp.c:750-757)
+        return (1, 16658, 64, -1075)
+    if name == "rev_expert":   # reversible with a budget, minbits padding and a precision cap
+        return (300, 2500, 40, -1075)
     raise ValueError(name)
 
 
-MODES = ["rate8", "rate21", "rate5_aligned", "prec16", "prec28", "acc1e-3", "acc1e-7", "expert"]
+MODES = ["rate8", "rate21", "rate5_aligned", "prec16", "prec28", "acc1e-3", "acc1e-7", "expert",
+         "reversible", "rev_expert"]
 
 
 def check(a, n, ztype, params, layout="as_is", header=0, device=True, device_stream=None):
@@ -64,9 +69,13 @@ def check(a, n, ztype, params, layout="as_is", header=0, device=True, device_str
         # serial decompression of the serial stream
         ref = np.zeros_like(base)
         hbits = run.header_bits
-        oracle_decompress(exp_words, ref, off, n, s, ztype, params, offset=hbits)
+        obits = oracle_decompress(exp_words, ref, off, n, s, ztype, params, offset=hbits)
         consumed, out = run.decompress(np.zeros_like(base))
-        assert consumed == size, f"decompress consumed {consumed} bytes, compress wrote {size}"
+        # zfp_decompress returns the aligned stream size after the last block
+        # read: the compressed size, except for reversible streams with
+        # minbits > 1, whose all-zero blocks are read as minbits bits
+        expect = (hbits + obits + 63) // 64 * 8
+        assert consumed == expect, f"decompress consumed {consumed} bytes, serial {expect} (compress wrote {size})"
         got_vals = read_layout(out, off, n, s)
         exp_vals = read_layout(ref, off, n, s)
         if not np.array_equal(got_vals.view(np.uint8), exp_vals.view(np.uint8)):
@@ -196,18 +205,33 @@ def test_variable_rate_without_index():
     assert api is not None
 
 
-def test_reversible_is_declined():
-    """Reversible mode is not offered by this policy: zfp_compress returns 0
-    and the stream is left untouched (the contract of
-    tests/src/endtoend/cudaExecBase.c:36-163 for unsupported requests)."""
-    n = [8, 8, 8]
-    a = smooth_field(n, T_DOUBLE, seed=1)
+@pytest.mark.parametrize("tname", list(TYPES))
+@pytest.mark.parametrize("kind", ["smooth", "noise", "specials", "zeros"])
+def test_reversible_is_lossless_and_serial(tname, kind):
+    """Reversible mode (src/template/rev*.c): the stream equals serial zfp's
+    and decompression returns every input bit (losslessness is what the mode
+    promises; the blocks cover both the block-floating-point and the
+    type-punned integer paths and all-zero blocks)."""
+    n = [10, 9, 7]
+    ztype = TYPES[tname]
+    a = smooth_field(n, ztype, seed=23, special=0.05 if kind == "specials" else 0.0)
+    if kind == "noise":
+        rng = np.random.default_rng(3)
+        a = rng.integers(0, 256, size=a.nbytes, dtype=np.uint8).view(a.dtype).copy()
+        if a.dtype.kind == "f":
+            a[~np.isfinite(a)] = 0   # NaN payloads are kept too, but keep the input plain
+    if kind == "zeros":
+        a[: a.size // 2] = 0
+        if a.dtype.kind == "f":
+            a[a.size // 2: a.size // 2 + 9] = -0.0
+    params = mode_params("reversible", ztype, 3)
+    check(a, n, ztype, params)
     base, off, s = make_layout(a, n, "as_is")
-    run = Run(base, off, n, s, T_DOUBLE, (1, 16658, 64, -1075))
+    run = Run(base, off, n, s, ztype, params)
     try:
-        assert run.compress() == 0
-        assert run.api.stream_wtell(run.bs) == 0
-        assert not run.dbuf.cpu().numpy().any()
+        run.compress()
+        _, out = run.decompress(np.zeros_like(base))
+        assert np.array_equal(out.view(np.uint8), base.view(np.uint8)), "reversible round trip lost bits"
     finally:
         run.close()
 

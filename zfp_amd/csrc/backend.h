@@ -31,6 +31,18 @@ int record_error(int code, const char* what);
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return record_hip_error(e_, #x); } while (0)
 
+// fixed-rate stream: every block exactly maxbits long (reversible streams,
+// minexp below ZFP_MIN_EXP, never are)
+inline bool fixed_rate(const Params& z) { return z.minbits == z.maxbits && z.minexp >= -1074; }
+
+// Whether the block offsets compression writes are where decompression
+// reads the blocks.  They are not for reversible streams with minbits > 1:
+// an all-zero float block is written as one bit and read as minbits bits
+// (src/template/revencodef.c:64-68, revdecodef.c:24-59), so serial
+// decompression places the following blocks elsewhere, and parity means
+// following it (the sequential parse does).
+inline bool index_matches_decoder(const Params& z) { return !(z.minexp < -1074 && z.minbits > 1); }
+
 // the library lock: every entry point that touches shared state holds it
 extern std::mutex g_mutex;
 // field descriptor from zfp-style extents and strides (0 = default)

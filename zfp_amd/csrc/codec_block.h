@@ -254,4 +254,226 @@ __device__ __forceinline__ u32 decode_block(BitReader& r, S* data, const Geom& g
   return hdr + c;
 }
 
+// ---------------------------------------------------------------------------
+// reversible (lossless) mode, selected by minexp < ZFP_MIN_EXP
+// (src/template/revencode.c, revencodef.c, revdecode.c, revdecodef.c)
+// ---------------------------------------------------------------------------
+
+// IEEE bits as two's-complement integers: negative values keep their sign
+// bit and have the magnitude bits flipped (revencodef.c:29-43); the map is
+// its own inverse (revdecodef.c:9-22)
+template <typename S>
+__device__ __forceinline__ typename Traits<S>::Int pun_float(S x)
+{
+  typedef typename Traits<S>::Int Int;
+  if constexpr (Traits<S>::P == 64) {
+    const Int b = (Int)__double_as_longlong(x);
+    return b < 0 ? (Int)((u64)b ^ 0x7fffffffffffffffull) : b;
+  }
+  else {
+    const Int b = (Int)__float_as_uint(x);
+    return b < 0 ? (Int)((u32)b ^ 0x7fffffffu) : b;
+  }
+}
+
+template <typename S>
+__device__ __forceinline__ S unpun_float(typename Traits<S>::Int b)
+{
+  if constexpr (Traits<S>::P == 64) {
+    const i64 x = b < 0 ? (i64)((u64)b ^ 0x7fffffffffffffffull) : b;
+    return __longlong_as_double((long long)x);
+  }
+  else {
+    const i32 x = b < 0 ? (i32)((u32)b ^ 0x7fffffffu) : b;
+    return __uint_as_float((u32)x);
+  }
+}
+
+// the integer part of a reversible block (revencode.c:36-79): Lorenzo
+// transform, reorder, then a precision of P minus the trailing zero planes
+// of all coefficients (1 at least, maxprec at most) in PBITS bits, then the
+// planes
+template <typename S, int D, typename W>
+__device__ __forceinline__ u32 rev_encode_ints(W& w, typename Traits<S>::Int* q, u32 minbits, u32 maxbits,
+                                               u32 maxprec)
+{
+  typedef Traits<S> T;
+  typedef typename T::Int Int;
+  typedef typename T::UInt UInt;
+  constexpr int N = 1 << (2 * D);
+  rev_fwd_xform<Int, D>(q);
+  UInt u[N];
+  UInt all = 0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    u[i] = ((UInt)q[order_at<D>(i)] + NB<UInt>::mask) ^ NB<UInt>::mask;
+    all |= u[i];
+  }
+  u32 prec = 0;
+  if (all)
+    prec = (u32)T::P - (u32)(T::P == 64 ? __builtin_ctzll((u64)all) : __builtin_ctz((u32)all));
+  prec = umin(prec, maxprec);
+  if (prec < 1)
+    prec = 1;
+  w.put(prec - 1, T::PBITS);
+  u32 c = T::PBITS;
+  if constexpr (D < 4)
+    c += encode_planes<N, T::P>(w, u, maxbits - T::PBITS, prec);
+  else
+    c += encode_planes256<UInt, T::P>(w, u, maxbits - T::PBITS, prec);
+  if (c < minbits) {
+    w.zeros(minbits - c);
+    c = minbits;
+  }
+  return c;
+}
+
+template <typename S, int D, typename W>
+__device__ __forceinline__ u32 encode_block_rev(W& w, const S* data, const Geom& g, const u32* bc,
+                                                const Params& z)
+{
+  typedef Traits<S> T;
+  typedef typename T::Int Int;
+  constexpr int N = 1 << (2 * D);
+  S v[N];
+  Int q[N];
+  gather<S, D>(v, data, g, bc);
+  if constexpr (T::FLOAT) {
+    // block-floating-point when the cast reproduces every value bit for bit
+    // (revencodef.c:5-15, 45-80); all-zero blocks reconstruct as +0
+    const int emax = block_emax<N>(v);
+    const bool zero = emax == -T::EBIAS;
+    S s, si;
+    if constexpr (T::P == 64) {
+      s = exp2_exact(62 - emax);
+      si = exp2_exact(emax - 62);
+    }
+    else {
+      s = exp2_exact_f(30 - emax);
+      si = exp2_exact_f(emax - 30);
+    }
+    bool exact = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      q[i] = zero ? (Int)0 : fwd_cast_one(v[i], s);
+      const S rec = zero ? (S)0 : inv_cast_one(q[i], si);
+      if constexpr (T::P == 64)
+        exact = exact && __double_as_longlong(rec) == __double_as_longlong(v[i]);
+      else
+        exact = exact && __float_as_uint(rec) == __float_as_uint(v[i]);
+    }
+    u32 hdr;
+    if (exact) {
+      const u32 e = (u32)(emax + T::EBIAS);
+      if (!e) {
+        // all-zero block: one zero bit, and no padding on this side
+        w.put(0, 1);
+        return 1;
+      }
+      w.put(1, 2);
+      w.put(e, T::EBITS);
+      hdr = 2 + T::EBITS;
+    }
+    else {
+#pragma unroll
+      for (int i = 0; i < N; i++)
+        q[i] = pun_float<S>(v[i]);
+      w.put(3, 2);
+      hdr = 2;
+    }
+    return hdr + rev_encode_ints<S, D>(w, q, z.minbits - umin(hdr, z.minbits), z.maxbits - hdr, z.maxprec);
+  }
+  else {
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      q[i] = (Int)v[i];
+    return rev_encode_ints<S, D>(w, q, z.minbits, z.maxbits, z.maxprec);
+  }
+}
+
+template <typename S, int D, bool STORE>
+__device__ __forceinline__ u32 rev_decode_ints(BitReader& r, typename Traits<S>::Int* q, u32 minbits,
+                                               u32 maxbits)
+{
+  typedef Traits<S> T;
+  typedef typename T::Int Int;
+  typedef typename T::UInt UInt;
+  constexpr int N = 1 << (2 * D);
+  const u32 prec = (u32)r.read(T::PBITS) + 1;
+  UInt u[N];
+  u32 c = T::PBITS;
+  if constexpr (D < 4)
+    c += decode_planes<N, T::P, UInt, STORE>(r, u, maxbits - T::PBITS, prec);
+  else
+    c += decode_planes256<UInt, T::P>(r, u, maxbits - T::PBITS, prec);
+  if (c < minbits)
+    c = minbits;
+  if (STORE) {
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      q[order_at<D>(i)] = (Int)((u[i] ^ NB<UInt>::mask) - NB<UInt>::mask);
+    rev_inv_xform<Int, D>(q);
+  }
+  return c;
+}
+
+template <typename S, int D, bool STORE>
+__device__ __forceinline__ u32 decode_block_rev(BitReader& r, S* data, const Geom& g, const u32* bc,
+                                                const Params& z)
+{
+  typedef Traits<S> T;
+  typedef typename T::Int Int;
+  constexpr int N = 1 << (2 * D);
+  Int q[N];
+  S v[N];
+  u32 c;
+  if constexpr (T::FLOAT) {
+    if (!r.read(1)) {
+      // all-zero block; the decoder skips to minbits (revdecodef.c:24-59)
+      if (STORE) {
+#pragma unroll
+        for (int i = 0; i < N; i++)
+          v[i] = (S)0;
+        scatter<S, D>(v, data, g, bc);
+      }
+      return z.minbits > 1 ? z.minbits : 1;
+    }
+    if (r.read(1)) {
+      c = 2 + rev_decode_ints<S, D, STORE>(r, q, z.minbits - umin(2, z.minbits), z.maxbits - 2);
+      if (STORE) {
+#pragma unroll
+        for (int i = 0; i < N; i++)
+          v[i] = unpun_float<S>(q[i]);
+      }
+    }
+    else {
+      const int emax = (int)r.read(T::EBITS) - T::EBIAS;
+      const u32 hdr = 2 + T::EBITS;
+      c = hdr + rev_decode_ints<S, D, STORE>(r, q, z.minbits - umin(hdr, z.minbits), z.maxbits - hdr);
+      if (STORE) {
+        S si;
+        if constexpr (T::P == 64)
+          si = exp2_exact(emax - 62);
+        else
+          si = exp2_exact_f(emax - 30);
+        const bool zero = emax == -T::EBIAS;
+#pragma unroll
+        for (int i = 0; i < N; i++)
+          v[i] = zero ? (S)0 : inv_cast_one(q[i], si);
+      }
+    }
+  }
+  else {
+    c = rev_decode_ints<S, D, STORE>(r, q, z.minbits, z.maxbits);
+    if (STORE) {
+#pragma unroll
+      for (int i = 0; i < N; i++)
+        v[i] = (S)q[i];
+    }
+  }
+  if (STORE)
+    scatter<S, D>(v, data, g, bc);
+  return c;
+}
+
 } // namespace zfphip

@@ -371,6 +371,58 @@ ZH_INLINE void inv_xform(Int* q)
   }
 }
 
+// reversible (Lorenzo) lifting in wrap-around arithmetic
+// (src/template/revencode.c:6-33, revdecode.c:6-30)
+template <typename Int>
+ZH_INLINE void rev_fwd_lift(Int& x, Int& y, Int& z, Int& w)
+{
+  typedef typename Traits<Int>::UInt U;
+  U a = (U)x, b = (U)y, c = (U)z, d = (U)w;
+  d -= c; c -= b; b -= a;
+  d -= c; c -= b;
+  d -= c;
+  x = (Int)a; y = (Int)b; z = (Int)c; w = (Int)d;
+}
+
+template <typename Int>
+ZH_INLINE void rev_inv_lift(Int& x, Int& y, Int& z, Int& w)
+{
+  typedef typename Traits<Int>::UInt U;
+  U a = (U)x, b = (U)y, c = (U)z, d = (U)w;
+  d += c;
+  c += b; d += c;
+  b += a; c += b; d += c;
+  x = (Int)a; y = (Int)b; z = (Int)c; w = (Int)d;
+}
+
+template <typename Int, int D>
+ZH_INLINE void rev_fwd_xform(Int* q)
+{
+  constexpr int N = 1 << (2 * D);
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const int st = 1 << (2 * a);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      if (((i >> (2 * a)) & 3) == 0)
+        rev_fwd_lift(q[i], q[i + st], q[i + 2 * st], q[i + 3 * st]);
+  }
+}
+
+template <typename Int, int D>
+ZH_INLINE void rev_inv_xform(Int* q)
+{
+  constexpr int N = 1 << (2 * D);
+#pragma unroll
+  for (int a = D - 1; a >= 0; a--) {
+    const int st = 1 << (2 * a);
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      if (((i >> (2 * a)) & 3) == 0)
+        rev_inv_lift(q[i], q[i + st], q[i + 2 * st], q[i + 3 * st]);
+  }
+}
+
 // --------------------------------------------------------------------------
 // bit-plane transpose: 32x32 bit matrices, least significant bit = column 0.
 // After transpose32(a), bit i of a[j] is bit j of the original a[i].

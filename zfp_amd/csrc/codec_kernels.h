@@ -29,7 +29,26 @@ namespace zfphip {
 // lane owns whole words and stores them without atomics
 enum { ENC_FIXED = 0, ENC_OFFSETS = 1, ENC_LENGTH = 2, ENC_FIXED_PLAIN = 3 };
 
-template <typename S, int D, int MODE>
+// the block codec of the stream's mode: lossy, or reversible (REV)
+template <typename S, int D, bool REV, typename W>
+__device__ __forceinline__ u32 encode_any(W& w, const S* data, const Geom& g, const u32* bc, const Params& z)
+{
+  if constexpr (REV)
+    return encode_block_rev<S, D>(w, data, g, bc, z);
+  else
+    return encode_block<S, D>(w, data, g, bc, z);
+}
+
+template <typename S, int D, bool STORE, bool REV>
+__device__ __forceinline__ u32 decode_any(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z)
+{
+  if constexpr (REV)
+    return decode_block_rev<S, D, STORE>(r, data, g, bc, z);
+  else
+    return decode_block<S, D, STORE>(r, data, g, bc, z);
+}
+
+template <typename S, int D, int MODE, bool REV = false>
 __global__ void __launch_bounds__(256, 2)
 k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
          u64* __restrict__ out, u64 offset, const u64* __restrict__ boffs, u64* __restrict__ lens)
@@ -42,7 +61,7 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
   if (MODE == ENC_LENGTH) {
     BitCounter cnt;
     cnt.pos = 0;
-    lens[b] = encode_block<S, D>(cnt, data, g, bc, z);
+    lens[b] = encode_any<S, D, REV>(cnt, data, g, bc, z);
     return;
   }
   u64 start, end;
@@ -56,7 +75,7 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
   }
   BitWriter<MODE == ENC_FIXED_PLAIN> w;
   w.begin(out, start, end);
-  encode_block<S, D>(w, data, g, bc, z);
+  encode_any<S, D, REV>(w, data, g, bc, z);
   w.finish();
 }
 
@@ -65,7 +84,7 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
 // flight at once, and the bit reader then refills from LDS.  Reading the
 // stream word by word from global memory inside the coder loop would wait
 // out the full memory latency at every refill.
-template <typename S, int D, bool STAGE>
+template <typename S, int D, bool STAGE, bool REV = false>
 __global__ void __launch_bounds__(256)
 k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restrict__ in,
          u64 nwords, u64 offset, const u64* __restrict__ boffs, u32 stride)
@@ -101,12 +120,12 @@ k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restr
   else {
     r.begin(in, nwords, start);
   }
-  decode_block<S, D, true>(r, data, g, bc, z);
+  decode_any<S, D, true, REV>(r, data, g, bc, z);
 }
 
 // One lane walks the stream and records every block's offset (relative to
 // `offset`); boffs receives nblocks + 1 entries.
-template <typename S, int D>
+template <typename S, int D, bool REV>
 __global__ void k_parse(Geom g, u32 nblocks, Params z, const u64* __restrict__ in, u64 nwords,
                         u64 offset, u64* __restrict__ boffs)
 {
@@ -118,12 +137,20 @@ __global__ void k_parse(Geom g, u32 nblocks, Params z, const u64* __restrict__ i
     boffs[b] = pos;
     BitReader r;
     r.begin(in, nwords, offset + pos);
-    pos += decode_block<S, D, false>(r, (S*)nullptr, g, bc, z);
+    pos += decode_any<S, D, false, REV>(r, (S*)nullptr, g, bc, z);
   }
   boffs[nblocks] = pos;
 }
 
 static inline unsigned grid_for(u64 n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// reversible mode (zfp_stream_set_reversible, src/zfp.c:750-757) is marked by
+// minexp below ZFP_MIN_EXP; its blocks have variable length
+static inline bool is_reversible(const Params& z) { return z.minexp < -1074; }
+
+template <typename S, int D, bool REV>
+int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits_out,
+                      hipStream_t st, u64** boffs_out);
 
 template <typename S, int D>
 int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
@@ -131,8 +158,12 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
 {
   const unsigned BS = 256;
   const unsigned grid = grid_for(d.nblocks, BS);
+  if (is_reversible(z))
+    return launch_encode_var<S, D, true>(d, data, z, out, offset, bits_out, st, boffs_out);
   const bool fixed = z.minbits == z.maxbits;
-  if (fixed) {
+  if (!fixed)
+    return launch_encode_var<S, D, false>(d, data, z, out, offset, bits_out, st, boffs_out);
+  {
     u64 total = (u64)d.nblocks * z.maxbits;
     u64 w0 = offset >> 6, w1 = (offset + total + 63) >> 6;
     if ((offset & 63) || (z.maxbits & 63)) {
@@ -152,7 +183,16 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
       *boffs_out = nullptr;
     return ZFP_HIP_OK;
   }
-  // variable rate: length pass, exclusive scan, prepare, write pass
+}
+
+// variable rate (and reversible): length pass, exclusive scan, prepare,
+// write pass
+template <typename S, int D, bool REV>
+int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits_out,
+                      hipStream_t st, u64** boffs_out)
+{
+  const unsigned BS = 256;
+  const unsigned grid = grid_for(d.nblocks, BS);
   u64 nb1 = (u64)d.nblocks + 1;
   void* lp;
   int rc = g_lens.get(2 * nb1 * sizeof(u64), &lp);
@@ -161,7 +201,7 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
   u64* lens = (u64*)lp;
   u64* offs = lens + nb1;
   HIPCHK(hipMemsetAsync(lens + d.nblocks, 0, sizeof(u64), st));
-  hipLaunchKernelGGL((k_encode<S, D, ENC_LENGTH>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+  hipLaunchKernelGGL((k_encode<S, D, ENC_LENGTH, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                      d.nblocks, z, out, offset, (const u64*)nullptr, lens);
   HIPCHK(hipGetLastError());
   rc = exclusive_scan(lens, offs, nb1, st);
@@ -174,7 +214,7 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
   if (w1 > w0)
     hipLaunchKernelGGL(k_prepare, dim3(grid_for(w1 - w0, 256)), dim3(256), 0, st, out, w0, w1,
                        (u32)(offset & 63));
-  hipLaunchKernelGGL((k_encode<S, D, ENC_OFFSETS>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+  hipLaunchKernelGGL((k_encode<S, D, ENC_OFFSETS, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                      d.nblocks, z, out, offset, (const u64*)offs, (u64*)nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
@@ -200,7 +240,8 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
 {
   const unsigned BS = 256;
   const unsigned grid = grid_for(d.nblocks, BS);
-  const bool fixed = z.minbits == z.maxbits;
+  const bool rev = is_reversible(z);
+  const bool fixed = !rev && z.minbits == z.maxbits;
   u64 total = 0;
   if (!fixed && !boffs) {
     // no index: recover block offsets by a sequential parse
@@ -208,8 +249,12 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
     int rc = g_lens.get(((u64)d.nblocks + 1) * sizeof(u64), &lp);
     if (rc)
       return rc;
-    hipLaunchKernelGGL((k_parse<S, D>), dim3(1), dim3(64), 0, st, d.g, d.nblocks, z, in, nwords, offset,
-                       (u64*)lp);
+    if (rev)
+      hipLaunchKernelGGL((k_parse<S, D, true>), dim3(1), dim3(64), 0, st, d.g, d.nblocks, z, in, nwords,
+                         offset, (u64*)lp);
+    else
+      hipLaunchKernelGGL((k_parse<S, D, false>), dim3(1), dim3(64), 0, st, d.g, d.nblocks, z, in, nwords,
+                         offset, (u64*)lp);
     HIPCHK(hipGetLastError());
     boffs = (const u64*)lp;
   }
@@ -224,6 +269,10 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
     hipLaunchKernelGGL((k_decode<S, D, true>), dim3(grid_for(d.nblocks, sbs)), dim3(sbs),
                        (size_t)sbs * stride * 8, st, (S*)data, d.g, d.nblocks, z, in, nwords, offset,
                        (const u64*)nullptr, stride);
+  }
+  else if (rev) {
+    hipLaunchKernelGGL((k_decode<S, D, false, true>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z,
+                       in, nwords, offset, boffs, 0u);
   }
   else {
     hipLaunchKernelGGL((k_decode<S, D, false>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z, in,

@@ -236,7 +236,11 @@ u64 max_block_bits(const Desc& d, const Params& z)
 {
   u32 vals = 1u << (2 * d.dims);
   u32 p = d.esize == 4 ? 32 : 64;
-  u32 hdr = d.type == zfp_type_float ? 9 : d.type == zfp_type_double ? 12 : 0;
+  // block headers: 1 + EBITS, or for reversible blocks 2 + EBITS + PBITS
+  // (floats) and PBITS (integers)
+  const bool rev = z.minexp < ZFP_MIN_EXP;
+  u32 hdr = d.type == zfp_type_float ? (rev ? 15 : 9) : d.type == zfp_type_double ? (rev ? 19 : 12)
+          : rev ? (d.esize == 4 ? 5 : 6) : 0;
   u64 mb = hdr + vals - 1 + (u64)vals * (z.maxprec < p ? z.maxprec : p);
   if (mb > z.maxbits)
     mb = z.maxbits;
@@ -256,8 +260,6 @@ struct StreamState {
 
 const char* params_problem(const zfp_stream* zs)
 {
-  if (zs->minexp < ZFP_MIN_EXP)
-    return "reversible mode is not offered by the HIP policy";
   if (zs->minbits > zs->maxbits || !zs->maxprec || zs->maxprec > 64)
     return "invalid compression parameters";
   return nullptr;
@@ -312,13 +314,13 @@ extern "C" int zfp_hip_encode(int type, const size_t n[4], const ptrdiff_t s[4],
   rc = make_desc(type, n, s, d);
   if (rc)
     return rc;
-  if (minexp < ZFP_MIN_EXP || minbits > maxbits || !maxprec || maxprec > 64 || !d_data || !d_words)
+  if (minbits > maxbits || !maxprec || maxprec > 64 || !d_data || !d_words)
     return record_error(ZFP_HIP_EINVAL, "invalid codec parameters or null pointer");
   Params z{minbits, maxbits, maxprec, minexp};
   u64 bits = 0;
   u64* boffs = nullptr;
   rc = dispatch_encode(d, d_data, z, d_words, offset, &bits, (hipStream_t)hip_stream,
-                       minbits == maxbits ? nullptr : &boffs);
+                       fixed_rate(z) || !index_matches_decoder(z) ? nullptr : &boffs);
   if (rc)
     return rc;
   if (boffs) {
@@ -351,11 +353,11 @@ extern "C" int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4],
   rc = make_desc(type, n, s, d);
   if (rc)
     return rc;
-  if (minexp < ZFP_MIN_EXP || minbits > maxbits || !maxprec || maxprec > 64 || !d_data || !d_words)
+  if (minbits > maxbits || !maxprec || maxprec > 64 || !d_data || !d_words)
     return record_error(ZFP_HIP_EINVAL, "invalid codec parameters or null pointer");
   Params z{minbits, maxbits, maxprec, minexp};
   const u64* boffs = nullptr;
-  if (minbits != maxbits) {
+  if (!fixed_rate(z) && index_matches_decoder(z)) {
     const Index* ix = index_find(d_words, offset, d, n, z);
     if (ix)
       boffs = ix->d_offsets;
@@ -430,8 +432,8 @@ extern "C" size_t zfp_hip_compress(zfp_stream* zs, const zfp_field* field)
     return hip_failed(e, "store of the buffered stream bits");
   u64 bits = 0;
   u64* boffs = nullptr;
-  const bool fixed = z.minbits == z.maxbits;
-  if (dispatch_encode(d, d_data, z, d_out, lead, &bits, st, fixed ? nullptr : &boffs))
+  const bool fixed = fixed_rate(z);
+  if (dispatch_encode(d, d_data, z, d_out, lead, &bits, st, fixed || !index_matches_decoder(z) ? nullptr : &boffs))
     return 0;
   const u64 words = (lead + bits + 63) / 64;
   if (!dev_stream && (e = hipMemcpy(bs->ptr, d_out, words * sizeof(u64), hipMemcpyDeviceToHost)) != hipSuccess) {
@@ -481,11 +483,11 @@ extern "C" size_t zfp_hip_decompress(zfp_stream* zs, zfp_field* field)
   const u64 word0 = rpos >> 6;
   const u32 lead = (u32)(rpos & 63);
   Params z{zs->minbits, zs->maxbits, zs->maxprec, zs->minexp};
-  const bool fixed = z.minbits == z.maxbits;
+  const bool fixed = fixed_rate(z);
   hipStream_t st = nullptr;
   hipError_t e;
 
-  const Index* ix = fixed ? nullptr : index_find(bs->begin, rpos, d, n, z);
+  const Index* ix = fixed || !index_matches_decoder(z) ? nullptr : index_find(bs->begin, rpos, d, n, z);
   const u64* boffs = ix ? ix->d_offsets : nullptr;
 
   // readable words: up to the end of the stream buffer, or of the stream
