@@ -231,19 +231,20 @@ ZH_INLINE void scatter(const S* v, S* data, const Geom& g, const u32* b)
 // 0 -> -EBIAS, subnormal -> 1-EBIAS, inf -> 0 (glibc frexp) -- the result of
 // src/template/encodef.c:10-40.  IEEE magnitudes order like their bit
 // patterns, so the max is an integer max of the sign-stripped bits.
+// The largest |x| is an IEEE maxNum reduction, which skips NaNs as the
+// reference's `max < fabs(x)` does; its biased exponent field ef then gives
+// the result (ef - 1022 for normal values).
 template <int N>
 ZH_INLINE int block_emax(const double* v)
 {
-  u64 mx = 0;
+  double mx = 0.0;
 #pragma unroll
-  for (int i = 0; i < N; i++) {
-    u64 a = (u64)__double_as_longlong(v[i]) & 0x7fffffffffffffffull;
-    a = a > 0x7ff0000000000000ull ? 0ull : a; // NaN never wins
-    mx = a > mx ? a : mx;
-  }
-  if (!mx)
+  for (int i = 0; i < N; i++)
+    mx = __builtin_fmax(mx, __builtin_fabs(v[i]));
+  const u64 bits = (u64)__double_as_longlong(mx);
+  if (!bits)
     return -1023;
-  int ef = (int)(mx >> 52);
+  const int ef = (int)(bits >> 52);
   if (ef == 2047)
     return 0;
   return ef == 0 ? -1022 : ef - 1022;
@@ -252,16 +253,14 @@ ZH_INLINE int block_emax(const double* v)
 template <int N>
 ZH_INLINE int block_emax(const float* v)
 {
-  u32 mx = 0;
+  float mx = 0.0f;
 #pragma unroll
-  for (int i = 0; i < N; i++) {
-    u32 a = (u32)__float_as_uint(v[i]) & 0x7fffffffu;
-    a = a > 0x7f800000u ? 0u : a;
-    mx = a > mx ? a : mx;
-  }
-  if (!mx)
+  for (int i = 0; i < N; i++)
+    mx = __builtin_fmaxf(mx, __builtin_fabsf(v[i]));
+  const u32 bits = __float_as_uint(mx);
+  if (!bits)
     return -127;
-  int ef = (int)(mx >> 23);
+  const int ef = (int)(bits >> 23);
   if (ef == 255)
     return 0;
   return ef == 0 ? -126 : ef - 126;
@@ -704,32 +703,47 @@ struct PlaneEncoder {
       pos += (u32)N;
       return;
     }
-    const u64 r = x >> n; // n < N <= 64
-    // group-test code of the remainder as a 128-bit word: bit 0 is the
-    // first group test; the j-th one-bit of r (at p) lands at p + j,
-    // followed by a 1 at p + j + 1 (the next group test)
-    u64 c0 = r ? 1ull : 0ull, c1 = 0;
-    u32 j = 0, p = 0;
-    for (u64 rr = r; rr; rr &= rr - 1) {
-      p = ctz64(rr);
-      j++;
-      xor128(c0, c1, 3ull, p + j);
-    }
-    u32 clen = 1; // a remainder without one-bits: a single 0 group test
     const u32 n0 = n;
-    if (r) {
-      if (p + 1 < (u32)N - n) {
-        // the group test after the last one-bit is a 0
-        xor128(c0, c1, 1ull, p + j + 1);
-        clen = p + j + 2;
+    const u64 r = x >> n0; // n0 < N <= 64
+    if (!r) {
+      // no new one-bit: the verbatim bits, then a 0 group test
+      w.put(x & lowmask(n0), n0 + 1);
+      pos += n0 + 1;
+      return;
+    }
+    // group-test code of the remainder: bit 0 is the first group test; the
+    // j-th one-bit of r (at p) lands at p + j, followed by a 1 at p + j + 1
+    // (the next group test).  With m one-bits, the highest at pm, the code
+    // has pm + m + 2 bits, or pm + m when pm is the last coefficient (its
+    // one-bit is implied and no group test follows it).
+    const u32 m = (u32)__builtin_popcountll(r);
+    const u32 pm = 63u - (u32)__builtin_clzll(r);
+    const bool implied = pm + 1 >= (u32)N - n0;
+    const u32 clen = implied ? pm + m : pm + m + 2;
+    n = n0 + pm + 1;
+    u64 c0 = 1, c1 = 0;
+    if (pm + m < 63) {
+      // the whole code lies below bit 64
+      u32 j = 0;
+      for (u64 rr = r; rr; rr &= rr - 1) {
+        j++;
+        c0 |= 3ull << (ctz64(rr) + j);
       }
-      else {
-        // the one-bit of the last coefficient is implied: neither it nor
-        // a group test follows the run
+      // drop the bits past the code: the group test after the last
+      // one-bit (a 0), or that one-bit too when it is implied
+      c0 &= lowmask(implied ? pm + m : pm + m + 1);
+    }
+    else {
+      u32 j = 0, p = 0;
+      for (u64 rr = r; rr; rr &= rr - 1) {
+        p = ctz64(rr);
+        j++;
         xor128(c0, c1, 3ull, p + j);
-        clen = p + j;
       }
-      n += p + 1;
+      if (implied)
+        xor128(c0, c1, 3ull, p + j);
+      else
+        xor128(c0, c1, 1ull, p + j + 1);
     }
     // the plane's code: the n0 verbatim bits, then the clen group-test
     // bits, appended as up to three words
@@ -786,23 +800,34 @@ struct PlaneDecoder {
     }
     // group tests: a 1, then the run of zeros up to the next one-bit; a
     // scan that reaches the last coefficient or the end of the budget
-    // stops there and that coefficient takes the one-bit
+    // stops there and that coefficient takes the one-bit.  One group test
+    // and its run take at most 64 bits; they are decoded from a local
+    // 64-bit window, which is refilled from the reader only when fewer
+    // bits than a test may need remain in it.
+    u64 w = r.peek64();
+    u32 used = 0;
     while (budget && n < (u32)N) {
-      const u64 w = r.peek64();
+      const u32 av = umin((u32)N - 1 - n, budget - 1); // <= 63: scan length
+      if (used + umin(av + 2, 64u) > 64) {
+        r.skip(used);
+        w = r.peek64();
+        used = 0;
+      }
+      const u64 wl = w >> used;
       budget--;
-      if (!(w & 1)) {
-        r.skip(1);
+      if (!(wl & 1)) {
+        used++;
         break;
       }
-      const u32 av = umin((u32)N - 1 - n, budget); // <= 63
-      const u32 p = ctz64(((w >> 1) & lowmask(av)) | (1ull << av));
+      const u32 p = ctz64(((wl >> 1) & lowmask(av)) | (1ull << av));
       const u32 hit = p < av ? 1u : 0u; // the run ends in a read one-bit
-      r.skip(1 + p + hit);
+      used += 1 + p + hit;
       budget -= p + hit;
       n += p;
       x |= 1ull << n;
       n++;
     }
+    r.skip(used);
     return x;
   }
 };

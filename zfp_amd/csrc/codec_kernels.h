@@ -48,8 +48,11 @@ __device__ __forceinline__ u32 decode_any(BitReader& r, S* data, const Geom& g, 
     return decode_block<S, D, STORE>(r, data, g, bc, z);
 }
 
-template <typename S, int D, int MODE, bool REV = false>
-__global__ void __launch_bounds__(256, 2)
+// OCC: waves per SIMD the register budget allows (2: up to 256 VGPRs, 3: up to
+// 168).  Three waves hide more latency when coding stays in the high half of
+// the planes; with the low half (high rates) the tighter budget spills.
+template <typename S, int D, int MODE, bool REV = false, int OCC = 2>
+__global__ void __launch_bounds__(256, OCC)
 k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
          u64* __restrict__ out, u64 offset, const u64* __restrict__ boffs, u64* __restrict__ lens)
 {
@@ -171,6 +174,12 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
                          (u32)(offset & 63));
       hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                          d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
+    }
+    else if (z.maxbits <= (16u << (2 * D)) && d.g.lax[0] == 0) {
+      // up to 16 bits per value with lanes along x (neighbouring lanes write
+      // neighbouring blocks): the three-wave variant
+      hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN, false, 3>), dim3(grid), dim3(BS), 0, st,
+                         (const S*)data, d.g, d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
     }
     else {
       hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN>), dim3(grid), dim3(BS), 0, st, (const S*)data,
