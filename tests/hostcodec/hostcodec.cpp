@@ -85,6 +85,37 @@ uint64_t dec(void* data, const size_t* n, const ptrdiff_t* s, const Params& z, c
   return pos - offset;
 }
 
+// the two-pass fixed-rate decode of 64-bit types: planes 63..32 of every
+// block, then every block that goes on below plane 32 in full
+template <typename S, int D>
+uint64_t dec_split(void* data, const size_t* n, const ptrdiff_t* s, const Params& z, const uint64_t* words,
+                   uint64_t nwords, uint64_t offset, uint64_t* nredo)
+{
+  Geom g;
+  u32 nb;
+  make_geom(D, n, s, g, nb);
+  std::vector<u32> redo;
+  for (u32 b = 0; b < nb; b++) {
+    u32 bc[4] = {0, 0, 0, 0};
+    block_coords<D>(b, g, bc);
+    BitReader r;
+    r.begin(words, nwords, offset + (uint64_t)b * z.maxbits);
+    bool more = false;
+    decode_block<S, D, true, true>(r, (S*)data, g, bc, z, &more);
+    if (more)
+      redo.push_back(b);
+  }
+  for (u32 b : redo) {
+    u32 bc[4] = {0, 0, 0, 0};
+    block_coords<D>(b, g, bc);
+    BitReader r;
+    r.begin(words, nwords, offset + (uint64_t)b * z.maxbits);
+    decode_block<S, D, true>(r, (S*)data, g, bc, z);
+  }
+  *nredo = redo.size();
+  return (uint64_t)nb * z.maxbits;
+}
+
 template <typename S>
 uint64_t enc_t(int dims, const void* data, const size_t* n, const ptrdiff_t* s, const Params& z,
                uint64_t* words, uint64_t offset)
@@ -137,4 +168,24 @@ extern "C" uint64_t hostcodec_decompress(int type, const size_t* n, const ptrdif
     case 3: return dec_t<float>(dims, data, n, s, z, words, nwords, offset);
     default: return dec_t<double>(dims, data, n, s, z, words, nwords, offset);
   }
+}
+
+// Fixed-rate int64/double fields of 1-3 dimensions through the two-pass
+// decode; *nredo receives the number of blocks decoded twice.
+extern "C" uint64_t hostcodec_decompress_split(int type, const size_t* n, const ptrdiff_t* s, void* data,
+                                               unsigned maxbits, unsigned maxprec, int minexp,
+                                               const uint64_t* words, uint64_t nwords, uint64_t offset,
+                                               uint64_t* nredo)
+{
+  int dims = n[0] ? n[1] ? n[2] ? n[3] ? 4 : 3 : 2 : 1 : 0;
+  Params z{maxbits, maxbits, maxprec, minexp};
+  if (dims < 1 || dims > 3 || (type != 2 && type != 4))
+    return 0;
+  if (type == 2)
+    return dims == 1 ? dec_split<i64, 1>(data, n, s, z, words, nwords, offset, nredo)
+         : dims == 2 ? dec_split<i64, 2>(data, n, s, z, words, nwords, offset, nredo)
+                     : dec_split<i64, 3>(data, n, s, z, words, nwords, offset, nredo);
+  return dims == 1 ? dec_split<double, 1>(data, n, s, z, words, nwords, offset, nredo)
+       : dims == 2 ? dec_split<double, 2>(data, n, s, z, words, nwords, offset, nredo)
+                   : dec_split<double, 3>(data, n, s, z, words, nwords, offset, nredo);
 }

@@ -242,3 +242,33 @@ def test_config0_fixed_rate_3d_float():
     n = [128, 128, 128]
     a = smooth_field(n, T_FLOAT, seed=2024)
     check(a, n, T_FLOAT, stream_params_rate(8, T_FLOAT, 3))
+
+
+def _split_field(n, seed):
+    """Smooth doubles with a constant block column and zero planes: at fixed
+    rates up to 16 bits per value the constant blocks code planes below 32
+    and the smooth ones stop above, so the 3D double decoder's second pass
+    (blocks that go below plane 32) has work."""
+    a = smooth_field(n, T_DOUBLE, seed=seed)
+    v = a.reshape(n[::-1])
+    v[..., :4] = v.flat[0]
+    v[..., -2:] = 0
+    return a
+
+
+@pytest.mark.parametrize("rate", [4, 8, 16])
+@pytest.mark.parametrize("layout", ["as_is", "permuted_neg"])
+@pytest.mark.parametrize("shape", ["3d", "3d_full"])
+def test_split_decode(shape, layout, rate):
+    n = SHAPES[shape]
+    a = _split_field(n, seed=zlib.crc32(f"split/{shape}".encode()) & 0xffff)
+    check(a, n, T_DOUBLE, stream_params_rate(rate, T_DOUBLE, 3, align=True), layout=layout)
+
+
+def test_split_decode_every_block_redone():
+    """A constant 256^3 double field at rate 16: every block codes planes
+    below 32, so all 262144 blocks take the second pass, two per lane of its
+    grid."""
+    n = [256, 256, 256]
+    a = np.full(n[0] * n[1] * n[2], 1.0 / 3.0)
+    check(a, n, T_DOUBLE, stream_params_rate(16, T_DOUBLE, 3, align=True))

@@ -100,9 +100,15 @@ __device__ __forceinline__ void decode_half(PlaneDecoder<N>& pd, BitReader& r, P
   }
 }
 
-template <int N, int P, typename UInt, bool STORE>
-__device__ __forceinline__ u32 decode_planes(BitReader& r, UInt* u, u32 maxbits, u32 maxprec)
+// HI_ONLY (64-bit coefficients): decode planes 63..32 only and set *more
+// when the block goes on below plane 32; such a block's coefficients lack
+// their low half and it has to be decoded again in full.  Leaving the low
+// half out frees the registers that hold it.
+template <int N, int P, typename UInt, bool STORE, bool HI_ONLY = false>
+__device__ __forceinline__ u32 decode_planes(BitReader& r, UInt* u, u32 maxbits, u32 maxprec,
+                                             bool* more = nullptr)
 {
+  static_assert(!HI_ONLY || P == 64, "HI_ONLY splits 64-bit coefficients");
   PlaneDecoder<N> pd;
   pd.init(P, maxbits, maxprec);
   u32 w[N];
@@ -114,6 +120,10 @@ __device__ __forceinline__ u32 decode_planes(BitReader& r, UInt* u, u32 maxbits,
 #pragma unroll
     for (int i = 0; i < N; i++)
       u[i] = STORE ? (UInt)w[i] << (P / 2) : 0;
+    if constexpr (HI_ONLY) {
+      *more = pd.wants(31);
+      return pd.used();
+    }
   }
   else {
 #pragma unroll
@@ -194,8 +204,9 @@ __device__ __forceinline__ u32 encode_block(W& w, const S* data, const Geom& g, 
 // decode one block (src/template/decodef.c:6-44, decodei.c:6-10,
 // decode.c:281-298); returns the block's length in bits
 // ---------------------------------------------------------------------------
-template <typename S, int D, bool STORE>
-__device__ __forceinline__ u32 decode_block(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z)
+template <typename S, int D, bool STORE, bool HI_ONLY = false>
+__device__ __forceinline__ u32 decode_block(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z,
+                                            bool* more = nullptr)
 {
   typedef Traits<S> T;
   typedef typename T::Int Int;
@@ -222,7 +233,7 @@ __device__ __forceinline__ u32 decode_block(BitReader& r, S* data, const Geom& g
   UInt u[N];
   u32 c;
   if constexpr (D < 4)
-    c = decode_planes<N, T::P, UInt, STORE>(r, u, maxbits, maxprec);
+    c = decode_planes<N, T::P, UInt, STORE, HI_ONLY>(r, u, maxbits, maxprec, more);
   else
     c = decode_planes256<UInt, T::P>(r, u, maxbits, maxprec);
   if (c < minbits)

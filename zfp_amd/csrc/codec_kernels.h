@@ -17,6 +17,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstring>
 
 #include "backend.h"
@@ -87,10 +88,17 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
 // flight at once, and the bit reader then refills from LDS.  Reading the
 // stream word by word from global memory inside the coder loop would wait
 // out the full memory latency at every refill.
-template <typename S, int D, bool STAGE, bool REV = false>
-__global__ void __launch_bounds__(256)
+//
+// HI_ONLY (fixed rate, 64-bit types): decode bit planes 63..32 only, which
+// fits three waves per SIMD instead of two, and append the lane numbers of
+// blocks that go on below plane 32 to redo[] (*nredo counts them, one atomic
+// per wave); k_decode_list then decodes those in full, over the partial
+// values written here.
+template <typename S, int D, bool STAGE, bool REV = false, bool HI_ONLY = false>
+__global__ void __launch_bounds__(256, HI_ONLY ? 3 : 2)
 k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restrict__ in,
-         u64 nwords, u64 offset, const u64* __restrict__ boffs, u32 stride)
+         u64 nwords, u64 offset, const u64* __restrict__ boffs, u32 stride,
+         u32* __restrict__ redo = nullptr, u32* __restrict__ nredo = nullptr)
 {
   extern __shared__ u64 stage[];
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,7 +131,43 @@ k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restr
   else {
     r.begin(in, nwords, start);
   }
-  decode_any<S, D, true, REV>(r, data, g, bc, z);
+  if constexpr (HI_ONLY) {
+    static_assert(!REV && Traits<S>::P == 64 && D < 4, "HI_ONLY: fixed-rate 64-bit types, 1-3D");
+    bool more = false;
+    decode_block<S, D, true, true>(r, data, g, bc, z, &more);
+    const u64 m = __ballot(more);
+    if (m) {
+      const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+      const u32 lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+      const u32 leader = (u32)__builtin_ctzll(m);
+      u32 base = 0;
+      if (lane == leader)
+        base = atomicAdd(nredo, (u32)__builtin_popcountll(m));
+      base = __builtin_amdgcn_readlane(base, leader);
+      if (more)
+        redo[base + below] = t;
+    }
+  }
+  else {
+    decode_any<S, D, true, REV>(r, data, g, bc, z);
+  }
+}
+
+// Full fixed-rate decode of the lanes listed in redo[0 .. *nredo), a
+// grid-stride loop over the list.
+template <typename S, int D>
+__global__ void __launch_bounds__(256)
+k_decode_list(S* __restrict__ data, Geom g, Params z, const u64* __restrict__ in, u64 nwords, u64 offset,
+              const u32* __restrict__ redo, const u32* __restrict__ nredo)
+{
+  const u32 n = *nredo;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    u32 bc[4] = {0, 0, 0, 0};
+    const u32 b = lane_block<D>(redo[i], g, bc);
+    BitReader r;
+    r.begin(in, nwords, offset + (u64)b * z.maxbits);
+    decode_block<S, D, true>(r, data, g, bc, z);
+  }
 }
 
 // One lane walks the stream and records every block's offset (relative to
@@ -274,7 +318,35 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
   unsigned sbs = 256;
   while (sbs > 64 && (u64)sbs * stride * 8 > 65536)
     sbs /= 2;
-  if (fixed && nwords > 0 && (u64)sbs * stride * 8 <= 65536) {
+  const bool staged = fixed && nwords > 0 && (u64)sbs * stride * 8 <= 65536;
+  if constexpr (std::is_same<S, double>::value && D == 3) {
+    // 3D doubles at up to 16 bits per value (the 1D and 2D decoders already
+    // run at five or more waves per SIMD): the cast puts each block's largest
+    // value at plane 62, so a block seldom reaches plane 31 (near-constant
+    // blocks do).  Decode the high planes at three waves per SIMD, then the
+    // few blocks that go lower.  (int64 values keep their own scale: small
+    // ones live in the low planes, so int64 keeps the one-pass decode.)
+    if (staged && z.maxbits <= (16u << (2 * D))) {
+      void* lp;
+      int rc = g_lens.get(((u64)d.nblocks + 1) * sizeof(u32), &lp);
+      if (rc)
+        return rc;
+      u32* nredo = (u32*)lp;
+      u32* redo = nredo + 1;
+      HIPCHK(hipMemsetAsync(nredo, 0, sizeof(u32), st));
+      hipLaunchKernelGGL((k_decode<S, D, true, false, true>), dim3(grid_for(d.nblocks, sbs)), dim3(sbs),
+                         (size_t)sbs * stride * 8, st, (S*)data, d.g, d.nblocks, z, in, nwords, offset,
+                         (const u64*)nullptr, stride, redo, nredo);
+      // 512 workgroups fill the chip at two waves per SIMD
+      hipLaunchKernelGGL((k_decode_list<S, D>), dim3(std::min(grid, 512u)), dim3(BS), 0, st, (S*)data, d.g, z,
+                         in, nwords, offset, (const u32*)redo, (const u32*)nredo);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st));
+      *bits_out = (u64)d.nblocks * z.maxbits;
+      return ZFP_HIP_OK;
+    }
+  }
+  if (staged) {
     hipLaunchKernelGGL((k_decode<S, D, true>), dim3(grid_for(d.nblocks, sbs)), dim3(sbs),
                        (size_t)sbs * stride * 8, st, (S*)data, d.g, d.nblocks, z, in, nwords, offset,
                        (const u64*)nullptr, stride);
