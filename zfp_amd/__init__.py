@@ -100,6 +100,8 @@ _API = {
     "zfp_field_metadata": (ctypes.c_uint64, [zf_p]),
     "zfp_field_set_metadata": (ctypes.c_int, [zf_p, ctypes.c_uint64]),
     "zfp_field_set_pointer": (None, [zf_p, ctypes.c_void_p]),
+    "zfp_field_set_type": (ctypes.c_int, [zf_p, ctypes.c_int]),
+    "zfp_field_type": (ctypes.c_int, [zf_p]),
     "zfp_field_set_stride_1d": (None, [zf_p, ctypes.c_ssize_t]),
     "zfp_field_set_stride_2d": (None, [zf_p, ctypes.c_ssize_t, ctypes.c_ssize_t]),
     "zfp_field_set_stride_3d": (None, [zf_p, ctypes.c_ssize_t, ctypes.c_ssize_t, ctypes.c_ssize_t]),
