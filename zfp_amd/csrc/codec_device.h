@@ -838,14 +838,54 @@ struct PlaneDecoder {
 // decode.c:133-184), which produce the same code as the N <= 64 variants.
 // The block stays in lane-private memory.
 // --------------------------------------------------------------------------
+// The bit planes of 256 coefficients as a table: X[4k + c] is plane k of
+// coefficients 64c .. 64c+63 (bit i = bit k of coefficient 64c + i).  Built
+// with the 32x32 transposes of planes_half, one 64-coefficient chunk and
+// one half at a time, so that the coder reads four words per plane instead
+// of extracting a bit from each of the 256 coefficients.
 template <typename UInt, int P>
-ZH_INLINE void plane256(const UInt* u, int k, u64* x)
+ZH_INLINE void planes256(const UInt* u, u64* X)
 {
+#pragma unroll
   for (int c = 0; c < 4; c++) {
-    u64 v = 0;
+    PlaneVec lo, hi;
+    planes_half<64, 0>(u + 64 * c, lo, hi);
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+      X[4 * k + c] = (u64)lo[k] | ((u64)hi[k] << 32);
+    if constexpr (P == 64) {
+      planes_half<64, 1>(u + 64 * c, lo, hi);
+#pragma unroll
+      for (int k = 0; k < 32; k++)
+        X[4 * (32 + k) + c] = (u64)lo[k] | ((u64)hi[k] << 32);
+    }
+  }
+}
+
+// inverse of planes256: coefficients from the plane table (P/32 halves)
+template <typename UInt, int P>
+ZH_INLINE void coeffs256(const u64* X, UInt* u)
+{
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+#pragma unroll
     for (int i = 0; i < 64; i++)
-      v |= (u64)((u[64 * c + i] >> k) & 1u) << i;
-    x[c] = v;
+      u[64 * c + i] = 0;
+#pragma unroll
+    for (int H = 0; H < P / 32; H++) {
+      PlaneVec lo, hi;
+#pragma unroll
+      for (int k = 0; k < 32; k++) {
+        const u64 x = X[4 * (32 * H + k) + c];
+        lo[k] = (u32)x;
+        hi[k] = (u32)(x >> 32);
+      }
+      u32 w[64];
+      coeffs_half<64>(lo, hi, w);
+#pragma unroll
+      for (int i = 0; i < 64; i++)
+        u[64 * c + i] |= (UInt)w[i] << (32 * H);
+    }
   }
 }
 
@@ -867,9 +907,13 @@ ZH_INLINE u32 encode_planes256(W& w, const UInt* u, u32 maxbits, u32 maxprec)
   u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
   const u32 budget0 = budget;
   u32 n = 0;
+  u64 X[4 * P];
+  planes256<UInt, P>(u, X);
   for (int k = P - 1; k >= (int)kmin && budget; k--) {
     u64 x[4];
-    plane256<UInt, P>(u, k, x);
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      x[c] = X[4 * k + c];
     // verbatim part, 64 bits at a time
     u32 m = umin(n, budget);
     budget -= m;
@@ -911,16 +955,20 @@ ZH_INLINE u32 decode_planes256(BitReader& r, UInt* u, u32 maxbits, u32 maxprec)
   u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
   const u32 budget0 = budget;
   u32 n = 0;
-  for (u32 i = 0; i < N; i++)
-    u[i] = 0;
+  // plane table as in planes256; planes that are not coded stay zero
+  u64 X[4 * P];
+  for (int i = 0; i < 4 * P; i++)
+    X[i] = 0;
   for (int k = P - 1; k >= (int)kmin && budget; k--) {
+    u64 x[4] = {0, 0, 0, 0};
     u32 m = umin(n, budget);
     budget -= m;
     for (u32 i = 0; i < m; i += 64) {
       u32 c = umin(64u, m - i);
-      u64 v = r.read(c);
-      for (u32 j = 0; j < c; j++)
-        u[i + j] += (UInt)((v >> j) & 1u) << k;
+      const u64 v = r.read(c);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        x[q] = (i >> 6) == (u32)q ? v : x[q];
     }
     while (budget && n < N) {
       budget--;
@@ -943,10 +991,17 @@ ZH_INLINE u32 decode_planes256(BitReader& r, UInt* u, u32 maxbits, u32 maxprec)
         budget -= av;
         n += av;
       }
-      u[n] += (UInt)1 << k;
+      const u64 bit = 1ull << (n & 63);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        x[q] |= (n >> 6) == (u32)q ? bit : 0ull;
       n++;
     }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      X[4 * k + q] = x[q];
   }
+  coeffs256<UInt, P>(X, u);
   return budget0 - budget;
 }
 

@@ -219,11 +219,13 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
       hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                          d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
     }
-    else if (z.maxbits <= (16u << (2 * D)) && d.g.lax[0] == 0) {
-      // up to 16 bits per value with lanes along x (neighbouring lanes write
-      // neighbouring blocks): the three-wave variant
-      hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN, false, 3>), dim3(grid), dim3(BS), 0, st,
-                         (const S*)data, d.g, d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
+    else if (D < 4 && z.maxbits <= (16u << (2 * D)) && d.g.lax[0] == 0) {
+      // 1-3D, up to 16 bits per value, lanes along x (neighbouring lanes
+      // write neighbouring blocks): the three-wave variant (4D blocks live
+      // in scratch, where a tighter register budget only adds spills)
+      if constexpr (D < 4)
+        hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN, false, 3>), dim3(grid), dim3(BS), 0, st,
+                           (const S*)data, d.g, d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
     }
     else {
       hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN>), dim3(grid), dim3(BS), 0, st, (const S*)data,
