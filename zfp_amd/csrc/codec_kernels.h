@@ -83,6 +83,125 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
   w.finish();
 }
 
+// Bit writer of the variable-length write pass whose wave assembles its
+// words in LDS: a wave's lanes hold consecutive blocks of the stream, so
+// their bits cover one contiguous range of words; words shared by two lanes
+// are OR-ed in LDS (cheap) instead of in global memory.  `staged` is the
+// same in every lane of the wave; when false (the wave's range is larger
+// than its LDS area) the writer stores to the stream directly, OR-ing
+// shared words with global atomics.
+struct WaveWriter {
+  u64* out;   // the stream
+  u64* st;    // the wave's LDS words; st[0] is stream word wbase
+  u64 wbase;
+  u64 wi;     // index of the word being filled
+  u64 acc;    // pending bits of word wi
+  u32 fill;   // number of pending bits (< 64)
+  u64 lo, hi; // the block's bit range
+  bool staged;
+
+  ZH_INLINE void emit(u64 word)
+  {
+    const u64 b0 = wi << 6;
+    if (b0 < hi) {
+      if (b0 + 64 > hi)
+        word &= lowmask((u32)(hi - b0));
+      const bool whole = b0 >= lo && b0 + 64 <= hi;
+      // two branches, so that each keeps its address space (LDS or global)
+      if (staged) {
+        if (whole)
+          st[wi - wbase] = word;
+        else if (word)
+          atomicOr((unsigned long long*)(st + (wi - wbase)), (unsigned long long)word);
+      }
+      else {
+        if (whole)
+          out[wi] = word;
+        else if (word)
+          atomicOr((unsigned long long*)(out + wi), (unsigned long long)word);
+      }
+    }
+    wi++;
+  }
+  ZH_INLINE void put(u64 v, u32 n)
+  {
+    acc |= v << fill;
+    u32 nf = fill + n;
+    if (nf >= 64) {
+      emit(acc);
+      acc = (v >> 1) >> (63 - fill);
+      nf -= 64;
+    }
+    fill = nf;
+  }
+  ZH_INLINE void zeros(u64 n)
+  {
+    while (n >= 64) { put(0, 64); n -= 64; }
+    if (n)
+      put(0, (u32)n);
+  }
+  ZH_INLINE void finish()
+  {
+    if (fill)
+      emit(acc);
+  }
+};
+
+// LDS operations of the lanes of one wave, ordered before what follows
+ZH_INLINE void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Write pass of variable-length coding when lane t holds stream block t (the
+// lane order is the stream's): each wave zeroes its LDS area (wcap words),
+// its lanes write their blocks there, then the wave copies the words to the
+// stream with whole-word stores; only the range's first and last words,
+// which neighbouring waves share, are OR-ed in global memory (into words
+// k_prepare zeroed).  boffs: block offsets relative to `offset`.
+template <typename S, int D, bool REV>
+__global__ void __launch_bounds__(256)
+k_encode_waves(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* __restrict__ out, u64 offset,
+               const u64* __restrict__ boffs, u32 wcap)
+{
+  extern __shared__ u64 wstage[];
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 lane = threadIdx.x & 63u;
+  const u32 t0 = t - lane;
+  if (t0 >= nblocks)
+    return; // the whole wave lies past the field
+  const u32 tl = umin(t0 + 63u, nblocks - 1u);
+  const u64 first = offset + boffs[t0], end = offset + boffs[tl + 1];
+  const u64 wlo = first >> 6, whi = (end + 63) >> 6;
+  const bool staged = whi - wlo <= wcap;
+  const u32 nw = staged ? (u32)(whi - wlo) : 0u;
+  u64* st = wstage + (u64)(threadIdx.x >> 6) * wcap;
+  for (u32 i = lane; i < nw; i += 64)
+    st[i] = 0;
+  wave_lds_sync();
+  if (t < nblocks) {
+    u32 bc[4] = {0, 0, 0, 0};
+    const u32 b = lane_block<D>(t, g, bc); // == t
+    WaveWriter w;
+    w.out = out; w.st = st; w.wbase = wlo; w.staged = staged;
+    w.lo = offset + boffs[b]; w.hi = offset + boffs[b + 1];
+    w.wi = w.lo >> 6; w.fill = (u32)(w.lo & 63); w.acc = 0;
+    encode_any<S, D, REV>(w, data, g, bc, z);
+    w.finish();
+  }
+  wave_lds_sync();
+  for (u32 i = lane; i < nw; i += 64) {
+    const u64 v = st[i];
+    const bool edge = (i == 0 && (first & 63)) || (i + 1 == nw && (end & 63));
+    if (!edge)
+      out[wlo + i] = v;
+    else if (v)
+      atomicOr((unsigned long long*)(out + wlo + i), (unsigned long long)v);
+  }
+}
+
 // STAGE (fixed rate, blocks of at most `stride` words): each lane first
 // copies its block's words into its own slot of LDS, with all loads in
 // flight at once, and the bit reader then refills from LDS.  Reading the
@@ -269,8 +388,22 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
   if (w1 > w0)
     hipLaunchKernelGGL(k_prepare, dim3(grid_for(w1 - w0, 256)), dim3(256), 0, st, out, w0, w1,
                        (u32)(offset & 63));
-  hipLaunchKernelGGL((k_encode<S, D, ENC_OFFSETS, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
-                     d.nblocks, z, out, offset, (const u64*)offs, (u64*)nullptr);
+  bool stream_order = true; // lane t holds stream block t
+  for (int i = 0; i < d.dims; i++)
+    stream_order = stream_order && d.g.lax[i] == (u32)i;
+  if (stream_order) {
+    // a wave's LDS area: twice the words 64 average blocks take, at least
+    // 64 and at most 2048 (16 KiB); a wave whose range is longer writes
+    // directly
+    const u64 avg_words = (total / (d.nblocks ? d.nblocks : 1) + 1) * 64 / 64;
+    const u32 wcap = (u32)std::min<u64>(2048, std::max<u64>(64, 2 * avg_words + 4));
+    hipLaunchKernelGGL((k_encode_waves<S, D, REV>), dim3(grid), dim3(BS), (size_t)4 * wcap * 8, st, (const S*)data,
+                       d.g, d.nblocks, z, out, offset, (const u64*)offs, wcap);
+  }
+  else {
+    hipLaunchKernelGGL((k_encode<S, D, ENC_OFFSETS, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+                       d.nblocks, z, out, offset, (const u64*)offs, (u64*)nullptr);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   *bits_out = total;
