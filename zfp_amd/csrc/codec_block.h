@@ -143,6 +143,156 @@ __device__ __forceinline__ u32 decode_planes(BitReader& r, UInt* u, u32 maxbits,
 }
 
 // ---------------------------------------------------------------------------
+// 4D blocks (256 values) in four w-slices of 64: the block exponent from a
+// first pass over the values; then per slice the cast and the lifts along
+// x, y and z in registers (the 3D code); then the lifts along w across the
+// slices, and the coefficients in zfp's order straight into the plane
+// table, one 64-coefficient chunk at a time.  Lifts along different axes
+// of different slices are independent, so this keeps the reference's order
+// x, y, z, w (encode4.c:41-65); decoding runs w, then z, y, x per slice
+// (decode4.c:27-52).  Only the w lifts, the reordering and the plane table
+// go through lane-private memory.
+// ---------------------------------------------------------------------------
+template <typename S, typename W>
+__device__ __forceinline__ u32 encode_block4(W& w, const S* data, const Geom& g, const u32* bc, const Params& z)
+{
+  typedef Traits<S> T;
+  typedef typename T::Int Int;
+  typedef typename T::UInt UInt;
+  u32 hdr = 0;
+  u32 minbits = z.minbits, maxbits = z.maxbits, maxprec = z.maxprec;
+  S s = 0;
+  if constexpr (T::FLOAT) {
+    // the largest |x| first, then its exponent: exponent(inf) is 0, so the
+    // slices' exponents cannot be combined instead
+    S mx = 0;
+#pragma unroll
+    for (int ws = 0; ws < 4; ws++) {
+      S v[64];
+      gather_slice4<S>(v, data, g, bc, ws);
+      const S m = max_abs<64>(v);
+      mx = mx > m ? mx : m; // both are NaN-free
+    }
+    const int emax = emax_of(mx);
+    const u32 mp = plane_count(emax, z.maxprec, z.minexp, 4);
+    const u32 e = mp ? (u32)(emax + T::EBIAS) : 0u;
+    hdr = e ? 1 + T::EBITS : 1;
+    w.put(e ? 2 * (u64)e + 1 : 0ull, hdr);
+    if constexpr (T::P == 64)
+      s = exp2_exact(62 - emax);
+    else
+      s = exp2_exact_f(30 - emax);
+    maxprec = e ? mp : 0u;
+    minbits = z.minbits - umin(hdr, z.minbits);
+    maxbits = z.maxbits - hdr;
+  }
+  Int q[256];
+#pragma unroll
+  for (int ws = 0; ws < 4; ws++) {
+    S v[64];
+    gather_slice4<S>(v, data, g, bc, ws);
+    Int t[64];
+#pragma unroll
+    for (int i = 0; i < 64; i++) {
+      if constexpr (T::FLOAT)
+        t[i] = fwd_cast_one(v[i], s);
+      else
+        t[i] = (Int)v[i];
+    }
+    fwd_xform<Int, 3>(t);
+#pragma unroll
+    for (int i = 0; i < 64; i++)
+      q[64 * ws + i] = t[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 64; i++)
+    fwd_lift(q[i], q[64 + i], q[128 + i], q[192 + i]);
+  u64 X[4 * T::P];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    UInt u[64];
+#pragma unroll
+    for (int i = 0; i < 64; i++)
+      u[i] = ((UInt)q[order_at<4>(64 * c + i)] + NB<UInt>::mask) ^ NB<UInt>::mask;
+    planes_chunk<UInt, T::P>(u, X, c);
+  }
+  u32 c = code_planes256<T::P>(w, X, maxbits, maxprec);
+  if (c < minbits) {
+    w.zeros(minbits - c);
+    c = minbits;
+  }
+  return hdr + c;
+}
+
+template <typename S, bool STORE>
+__device__ __forceinline__ u32 decode_block4(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z)
+{
+  typedef Traits<S> T;
+  typedef typename T::Int Int;
+  typedef typename T::UInt UInt;
+  u32 hdr = 0;
+  u32 minbits = z.minbits, maxbits = z.maxbits, maxprec = z.maxprec;
+  int emax = 0;
+  if constexpr (T::FLOAT) {
+    const bool coded = r.read(1) != 0;
+    if (coded) {
+      emax = (int)r.read(T::EBITS) - T::EBIAS;
+      hdr = 1 + T::EBITS;
+      maxprec = plane_count(emax, z.maxprec, z.minexp, 4);
+    }
+    else {
+      hdr = 1;
+      maxprec = 0;
+    }
+    minbits = z.minbits - umin(hdr, z.minbits);
+    maxbits = z.maxbits - hdr;
+  }
+  u64 X[4 * T::P];
+  u32 c = decode_table256<T::P>(r, X, maxbits, maxprec);
+  if (c < minbits)
+    c = minbits;
+  if (STORE) {
+    Int q[256];
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+      UInt u[64];
+      coeffs_chunk<UInt, T::P>(X, ch, u);
+#pragma unroll
+      for (int i = 0; i < 64; i++)
+        q[order_at<4>(64 * ch + i)] = (Int)((u[i] ^ NB<UInt>::mask) - NB<UInt>::mask);
+    }
+#pragma unroll
+    for (int i = 0; i < 64; i++)
+      inv_lift(q[i], q[64 + i], q[128 + i], q[192 + i]);
+    S sc = 0;
+    if constexpr (T::FLOAT) {
+      if constexpr (T::P == 64)
+        sc = exp2_exact(emax - 62);
+      else
+        sc = exp2_exact_f(emax - 30);
+    }
+#pragma unroll
+    for (int ws = 0; ws < 4; ws++) {
+      Int t[64];
+#pragma unroll
+      for (int i = 0; i < 64; i++)
+        t[i] = q[64 * ws + i];
+      inv_xform<Int, 3>(t);
+      S v[64];
+#pragma unroll
+      for (int i = 0; i < 64; i++) {
+        if constexpr (T::FLOAT)
+          v[i] = inv_cast_one(t[i], sc);
+        else
+          v[i] = (S)t[i];
+      }
+      scatter_slice4<S>(v, data, g, bc, ws);
+    }
+  }
+  return hdr + c;
+}
+
+// ---------------------------------------------------------------------------
 // encode one block (src/template/encodef.c:62-90, encodei.c:6-15,
 // encode.c:259-280); returns the block's length in bits.  A float block
 // with nothing to code (header bit 0) runs the same path with no planes, so
@@ -151,6 +301,8 @@ __device__ __forceinline__ u32 decode_planes(BitReader& r, UInt* u, u32 maxbits,
 template <typename S, int D, typename W>
 __device__ __forceinline__ u32 encode_block(W& w, const S* data, const Geom& g, const u32* bc, const Params& z)
 {
+  if constexpr (D == 4)
+    return encode_block4<S>(w, data, g, bc, z);
   typedef Traits<S> T;
   typedef typename T::Int Int;
   typedef typename T::UInt UInt;
@@ -208,6 +360,8 @@ template <typename S, int D, bool STORE, bool HI_ONLY = false>
 __device__ __forceinline__ u32 decode_block(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z,
                                             bool* more = nullptr)
 {
+  if constexpr (D == 4)
+    return decode_block4<S, STORE>(r, data, g, bc, z);
   typedef Traits<S> T;
   typedef typename T::Int Int;
   typedef typename T::UInt UInt;

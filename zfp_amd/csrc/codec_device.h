@@ -223,6 +223,54 @@ ZH_INLINE void scatter(const S* v, S* data, const Geom& g, const u32* b)
   }
 }
 
+// The 64 values of w-slice ws (0..3) of a 4D block, with the padding of
+// gather<S,4> (the slice's w position comes from the w axis's offset table)
+template <typename S>
+ZH_INLINE void gather_slice4(S* v, const S* data, const Geom& g, const u32* b, int ws)
+{
+  i64 o[4][4];
+  const S* p = data;
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const u32 c = 4 * b[d];
+    p += (i64)c * g.s[d];
+    axis_offsets(g.s[d], umin(4u, g.n[d] - c), o[d]);
+  }
+  p += o[3][ws];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const S* q = p + o[1][r & 3] + o[2][r >> 2];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+      v[4 * r + x] = q[o[0][x]];
+  }
+}
+
+// Write back the valid part of w-slice ws of a 4D block (scatter<S,4>)
+template <typename S>
+ZH_INLINE void scatter_slice4(const S* v, S* data, const Geom& g, const u32* b, int ws)
+{
+  u32 m[4];
+  S* p = data;
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const u32 o = 4 * b[d];
+    m[d] = umin(4u, g.n[d] - o);
+    p += (i64)o * g.s[d];
+  }
+  p += (i64)ws * g.s[3];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const u32 y = r & 3, z = r >> 2;
+    const bool ok = (u32)ws < m[3] && y < m[1] && z < m[2];
+    S* q = p + (i64)y * g.s[1] + (i64)z * g.s[2];
+#pragma unroll
+    for (u32 x = 0; x < 4; x++)
+      if (ok && x < m[0])
+        q[(i64)x * g.s[0]] = v[4 * r + x];
+  }
+}
+
 // --------------------------------------------------------------------------
 // block-floating-point cast, computed on the IEEE bit patterns
 // --------------------------------------------------------------------------
@@ -235,12 +283,28 @@ ZH_INLINE void scatter(const S* v, S* data, const Geom& g, const u32* b)
 // reference's `max < fabs(x)` does; its biased exponent field ef then gives
 // the result (ef - 1022 for normal values).
 template <int N>
-ZH_INLINE int block_emax(const double* v)
+ZH_INLINE double max_abs(const double* v)
 {
   double mx = 0.0;
 #pragma unroll
   for (int i = 0; i < N; i++)
     mx = __builtin_fmax(mx, __builtin_fabs(v[i]));
+  return mx;
+}
+
+template <int N>
+ZH_INLINE float max_abs(const float* v)
+{
+  float mx = 0.0f;
+#pragma unroll
+  for (int i = 0; i < N; i++)
+    mx = __builtin_fmaxf(mx, __builtin_fabsf(v[i]));
+  return mx;
+}
+
+// the block exponent of a block whose largest |x| (NaNs skipped) is mx
+ZH_INLINE int emax_of(double mx)
+{
   const u64 bits = (u64)__double_as_longlong(mx);
   if (!bits)
     return -1023;
@@ -250,13 +314,8 @@ ZH_INLINE int block_emax(const double* v)
   return ef == 0 ? -1022 : ef - 1022;
 }
 
-template <int N>
-ZH_INLINE int block_emax(const float* v)
+ZH_INLINE int emax_of(float mx)
 {
-  float mx = 0.0f;
-#pragma unroll
-  for (int i = 0; i < N; i++)
-    mx = __builtin_fmaxf(mx, __builtin_fabsf(v[i]));
   const u32 bits = __float_as_uint(mx);
   if (!bits)
     return -127;
@@ -265,6 +324,9 @@ ZH_INLINE int block_emax(const float* v)
     return 0;
   return ef == 0 ? -126 : ef - 126;
 }
+
+template <int N> ZH_INLINE int block_emax(const double* v) { return emax_of(max_abs<N>(v)); }
+template <int N> ZH_INLINE int block_emax(const float* v) { return emax_of(max_abs<N>(v)); }
 
 // 2^e as a double, exactly as ldexp(1.0, e) rounds it (0 below 2^-1074,
 // +inf from 2^1024 up)
@@ -844,49 +906,59 @@ struct PlaneDecoder {
 // one half at a time, so that the coder reads four words per plane instead
 // of extracting a bit from each of the 256 coefficients.
 template <typename UInt, int P>
-ZH_INLINE void planes256(const UInt* u, u64* X)
+ZH_INLINE void planes_chunk(const UInt* u, u64* X, int c)
 {
+  PlaneVec lo, hi;
+  planes_half<64, 0>(u, lo, hi);
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    PlaneVec lo, hi;
-    planes_half<64, 0>(u + 64 * c, lo, hi);
+  for (int k = 0; k < 32; k++)
+    X[4 * k + c] = (u64)lo[k] | ((u64)hi[k] << 32);
+  if constexpr (P == 64) {
+    planes_half<64, 1>(u, lo, hi);
 #pragma unroll
     for (int k = 0; k < 32; k++)
-      X[4 * k + c] = (u64)lo[k] | ((u64)hi[k] << 32);
-    if constexpr (P == 64) {
-      planes_half<64, 1>(u + 64 * c, lo, hi);
-#pragma unroll
-      for (int k = 0; k < 32; k++)
-        X[4 * (32 + k) + c] = (u64)lo[k] | ((u64)hi[k] << 32);
-    }
+      X[4 * (32 + k) + c] = (u64)lo[k] | ((u64)hi[k] << 32);
   }
 }
 
-// inverse of planes256: coefficients from the plane table (P/32 halves)
+template <typename UInt, int P>
+ZH_INLINE void planes256(const UInt* u, u64* X)
+{
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    planes_chunk<UInt, P>(u + 64 * c, X, c);
+}
+
+// inverse of planes_chunk: coefficients 64c .. 64c+63 from the plane table
+template <typename UInt, int P>
+ZH_INLINE void coeffs_chunk(const u64* X, int c, UInt* u)
+{
+#pragma unroll
+  for (int i = 0; i < 64; i++)
+    u[i] = 0;
+#pragma unroll
+  for (int H = 0; H < P / 32; H++) {
+    PlaneVec lo, hi;
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      const u64 x = X[4 * (32 * H + k) + c];
+      lo[k] = (u32)x;
+      hi[k] = (u32)(x >> 32);
+    }
+    u32 w[64];
+    coeffs_half<64>(lo, hi, w);
+#pragma unroll
+    for (int i = 0; i < 64; i++)
+      u[i] |= (UInt)w[i] << (32 * H);
+  }
+}
+
 template <typename UInt, int P>
 ZH_INLINE void coeffs256(const u64* X, UInt* u)
 {
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-#pragma unroll
-    for (int i = 0; i < 64; i++)
-      u[64 * c + i] = 0;
-#pragma unroll
-    for (int H = 0; H < P / 32; H++) {
-      PlaneVec lo, hi;
-#pragma unroll
-      for (int k = 0; k < 32; k++) {
-        const u64 x = X[4 * (32 * H + k) + c];
-        lo[k] = (u32)x;
-        hi[k] = (u32)(x >> 32);
-      }
-      u32 w[64];
-      coeffs_half<64>(lo, hi, w);
-#pragma unroll
-      for (int i = 0; i < 64; i++)
-        u[64 * c + i] |= (UInt)w[i] << (32 * H);
-    }
-  }
+  for (int c = 0; c < 4; c++)
+    coeffs_chunk<UInt, P>(X, c, u + 64 * c);
 }
 
 // bits of the 256-bit plane x starting at position n (n < 256), 64 at most
@@ -899,16 +971,15 @@ ZH_INLINE u64 bits256(const u64* x, u32 n)
   return v;
 }
 
-template <typename UInt, int P, typename W>
-ZH_INLINE u32 encode_planes256(W& w, const UInt* u, u32 maxbits, u32 maxprec)
+// the embedded coder over a plane table (planes_chunk layout)
+template <int P, typename W>
+ZH_INLINE u32 code_planes256(W& w, const u64* X, u32 maxbits, u32 maxprec)
 {
   const u32 N = 256;
   const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
   u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
   const u32 budget0 = budget;
   u32 n = 0;
-  u64 X[4 * P];
-  planes256<UInt, P>(u, X);
   for (int k = P - 1; k >= (int)kmin && budget; k--) {
     u64 x[4];
 #pragma unroll
@@ -947,16 +1018,24 @@ ZH_INLINE u32 encode_planes256(W& w, const UInt* u, u32 maxbits, u32 maxprec)
   return budget0 - budget;
 }
 
-template <typename UInt, int P>
-ZH_INLINE u32 decode_planes256(BitReader& r, UInt* u, u32 maxbits, u32 maxprec)
+template <typename UInt, int P, typename W>
+ZH_INLINE u32 encode_planes256(W& w, const UInt* u, u32 maxbits, u32 maxprec)
+{
+  u64 X[4 * P];
+  planes256<UInt, P>(u, X);
+  return code_planes256<P>(w, X, maxbits, maxprec);
+}
+
+// the decoder into a plane table (planes_chunk layout); planes that are not
+// coded stay zero.  Returns the bits consumed.
+template <int P>
+ZH_INLINE u32 decode_table256(BitReader& r, u64* X, u32 maxbits, u32 maxprec)
 {
   const u32 N = 256;
   const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
   u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
   const u32 budget0 = budget;
   u32 n = 0;
-  // plane table as in planes256; planes that are not coded stay zero
-  u64 X[4 * P];
   for (int i = 0; i < 4 * P; i++)
     X[i] = 0;
   for (int k = P - 1; k >= (int)kmin && budget; k--) {
@@ -1001,8 +1080,16 @@ ZH_INLINE u32 decode_planes256(BitReader& r, UInt* u, u32 maxbits, u32 maxprec)
     for (int q = 0; q < 4; q++)
       X[4 * k + q] = x[q];
   }
-  coeffs256<UInt, P>(X, u);
   return budget0 - budget;
+}
+
+template <typename UInt, int P>
+ZH_INLINE u32 decode_planes256(BitReader& r, UInt* u, u32 maxbits, u32 maxprec)
+{
+  u64 X[4 * P];
+  const u32 c = decode_table256<P>(r, X, maxbits, maxprec);
+  coeffs256<UInt, P>(X, u);
+  return c;
 }
 
 } // namespace zfphip
