@@ -82,6 +82,33 @@ int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4], void* d_da
                    const uint64_t* d_words, uint64_t offset, uint64_t* bits_out,
                    void* hip_stream);
 
+/*
+ * Block codec for compressed arrays: replaces, for many blocks in one
+ * launch, the per-block calls of zfp::codec::zfp_base::encode_block and
+ * decode_block (include/zfp/codec/zfpcodec.hpp:146-190), which the
+ * reference's compressed arrays make one cached block at a time.
+ * d_blocks holds `count` whole blocks of 4^dims values, one after another
+ * (block k at 4^dims * k values, x fastest within a block), in device
+ * memory.  d_words/nwords: the device stream and its size in words.
+ *   d_slots NULL: block k is block k of a stream that starts at bit
+ *     `offset`: the stream zfp_hip_encode writes for the blocks as a field
+ *     of extents 4 x ... x 4*count.  Any mode; variable-length streams have
+ *     their block index recorded as by zfp_hip_encode.
+ *   d_slots non-NULL (device array of count slot numbers): fixed rate with
+ *     maxbits and offset multiples of 64 (the rates compressed arrays use);
+ *     block k is written to / read from bits offset + d_slots[k] * maxbits,
+ *     and no other word of the stream is touched.  A slot whose block would
+ *     end past nwords fails the call (ZFP_HIP_EINVAL) before any block is
+ *     coded.
+ * *bits_out: bits written (consumed); with slots, count * maxbits.
+ */
+int zfp_hip_encode_blocks(int type, unsigned dims, size_t count, const void* d_blocks, const uint64_t* d_slots,
+                          uint minbits, uint maxbits, uint maxprec, int minexp, uint64_t* d_words,
+                          uint64_t nwords, uint64_t offset, uint64_t* bits_out, void* hip_stream);
+int zfp_hip_decode_blocks(int type, unsigned dims, size_t count, void* d_blocks, const uint64_t* d_slots,
+                          uint minbits, uint maxbits, uint maxprec, int minexp, const uint64_t* d_words,
+                          uint64_t nwords, uint64_t offset, uint64_t* bits_out, void* hip_stream);
+
 /* Number of per-block bit offsets recorded for the stream whose buffer
    begins at `stream_begin` (0 if none); and export/import of that index as
    64-bit absolute bit offsets, one per block plus the end offset, so that a

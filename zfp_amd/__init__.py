@@ -155,6 +155,14 @@ _HIP = {
     "zfp_hip_decode": (ctypes.c_int, [ctypes.c_int, _SZ4, _PD4, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
                                       ctypes.c_uint, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
                                       _P(ctypes.c_uint64), ctypes.c_void_p]),
+    "zfp_hip_encode_blocks": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint, ctypes.c_size_t, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                             _P(ctypes.c_uint64), ctypes.c_void_p]),
+    "zfp_hip_decode_blocks": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint, ctypes.c_size_t, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                             _P(ctypes.c_uint64), ctypes.c_void_p]),
     "zfp_hip_index_size": (ctypes.c_size_t, [ctypes.c_void_p]),
     "zfp_hip_index_export": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), ctypes.c_size_t]),
     "zfp_hip_index_import": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), ctypes.c_size_t]),

@@ -94,15 +94,19 @@ extern Scratch g_lens, g_scan_tmp;
 // caller must hipFree.
 template <int D>
 int dispatch_encode_d(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
-                      u64* bits, hipStream_t st, u64** boffs);
+                      u64* bits, hipStream_t st, u64** boffs, const u64* at = nullptr);
 template <int D>
 int dispatch_decode_d(const Desc& d, void* data, const Params& z, const u64* in, u64 nwords,
                       u64 offset, const u64* boffs, u64* bits, hipStream_t st);
 
-extern template int dispatch_encode_d<1>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**);
-extern template int dispatch_encode_d<2>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**);
-extern template int dispatch_encode_d<3>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**);
-extern template int dispatch_encode_d<4>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**);
+extern template int dispatch_encode_d<1>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
+                                           const u64*);
+extern template int dispatch_encode_d<2>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
+                                           const u64*);
+extern template int dispatch_encode_d<3>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
+                                           const u64*);
+extern template int dispatch_encode_d<4>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
+                                           const u64*);
 extern template int dispatch_decode_d<1>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
 extern template int dispatch_decode_d<2>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
 extern template int dispatch_decode_d<3>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);

@@ -70,7 +70,8 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
   }
   u64 start, end;
   if (MODE == ENC_FIXED || MODE == ENC_FIXED_PLAIN) {
-    start = offset + (u64)b * z.maxbits;
+    // block b's place: b * maxbits, or boffs[b] when the caller chose it
+    start = offset + (boffs ? boffs[b] : (u64)b * z.maxbits);
     end = start + z.maxbits;
   }
   else {
@@ -318,9 +319,11 @@ template <typename S, int D, bool REV>
 int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits_out,
                       hipStream_t st, u64** boffs_out);
 
+// at (fixed rate, word-aligned blocks only): block b is written at bit
+// offset + at[b] instead of offset + b * maxbits, and no other word is touched
 template <typename S, int D>
 int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
-                         u64* bits_out, hipStream_t st, u64** boffs_out)
+                         u64* bits_out, hipStream_t st, u64** boffs_out, const u64* at)
 {
   const unsigned BS = 256;
   const unsigned grid = grid_for(d.nblocks, BS);
@@ -329,6 +332,18 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
   const bool fixed = z.minbits == z.maxbits;
   if (!fixed)
     return launch_encode_var<S, D, false>(d, data, z, out, offset, bits_out, st, boffs_out);
+  if (at) {
+    if ((offset & 63) || (z.maxbits & 63))
+      return record_error(ZFP_HIP_EINVAL, "blocks at chosen places need word-aligned blocks");
+    hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+                       d.nblocks, z, out, offset, at, (u64*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    *bits_out = (u64)d.nblocks * z.maxbits;
+    if (boffs_out)
+      *boffs_out = nullptr;
+    return ZFP_HIP_OK;
+  }
   {
     u64 total = (u64)d.nblocks * z.maxbits;
     u64 w0 = offset >> 6, w1 = (offset + total + 63) >> 6;
@@ -461,7 +476,7 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
     // blocks do).  Decode the high planes at three waves per SIMD, then the
     // few blocks that go lower.  (int64 values keep their own scale: small
     // ones live in the low planes, so int64 keeps the one-pass decode.)
-    if (staged && z.maxbits <= (16u << (2 * D))) {
+    if (staged && !boffs && z.maxbits <= (16u << (2 * D))) {
       void* lp;
       int rc = g_lens.get(((u64)d.nblocks + 1) * sizeof(u32), &lp);
       if (rc)
@@ -482,9 +497,10 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
     }
   }
   if (staged) {
+    // fixed rate: boffs, when given, places block b at offset + boffs[b]
     hipLaunchKernelGGL((k_decode<S, D, true>), dim3(grid_for(d.nblocks, sbs)), dim3(sbs),
                        (size_t)sbs * stride * 8, st, (S*)data, d.g, d.nblocks, z, in, nwords, offset,
-                       (const u64*)nullptr, stride);
+                       boffs, stride);
   }
   else if (rev) {
     hipLaunchKernelGGL((k_decode<S, D, false, true>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z,
@@ -492,7 +508,7 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
   }
   else {
     hipLaunchKernelGGL((k_decode<S, D, false>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z, in,
-                       nwords, offset, fixed ? (const u64*)nullptr : boffs, 0u);
+                       nwords, offset, boffs, 0u);
   }
   HIPCHK(hipGetLastError());
   if (fixed)
@@ -506,13 +522,13 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
 
 template <int D>
 int dispatch_encode_d(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
-                             u64* bits, hipStream_t st, u64** boffs)
+                             u64* bits, hipStream_t st, u64** boffs, const u64* at)
 {
   switch (d.type) {
-    case zfp_type_int32: return launch_encode<i32, D>(d, data, z, out, offset, bits, st, boffs);
-    case zfp_type_int64: return launch_encode<i64, D>(d, data, z, out, offset, bits, st, boffs);
-    case zfp_type_float: return launch_encode<float, D>(d, data, z, out, offset, bits, st, boffs);
-    case zfp_type_double: return launch_encode<double, D>(d, data, z, out, offset, bits, st, boffs);
+    case zfp_type_int32: return launch_encode<i32, D>(d, data, z, out, offset, bits, st, boffs, at);
+    case zfp_type_int64: return launch_encode<i64, D>(d, data, z, out, offset, bits, st, boffs, at);
+    case zfp_type_float: return launch_encode<float, D>(d, data, z, out, offset, bits, st, boffs, at);
+    case zfp_type_double: return launch_encode<double, D>(d, data, z, out, offset, bits, st, boffs, at);
   }
   return ZFP_HIP_EINVAL;
 }

@@ -2,6 +2,7 @@
 #include "codec_kernels.h"
 
 namespace zfphip {
-template int dispatch_encode_d<1>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**);
+template int dispatch_encode_d<1>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
+                                    const u64*);
 template int dispatch_decode_d<1>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
 } // namespace zfphip

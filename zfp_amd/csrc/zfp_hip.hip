@@ -96,13 +96,13 @@ int make_desc(int type, const size_t* n, const ptrdiff_t* s, Desc& d)
 }
 
 int dispatch_encode(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits,
-                    hipStream_t st, u64** boffs)
+                    hipStream_t st, u64** boffs, const u64* at = nullptr)
 {
   switch (d.dims) {
-    case 1: return dispatch_encode_d<1>(d, data, z, out, offset, bits, st, boffs);
-    case 2: return dispatch_encode_d<2>(d, data, z, out, offset, bits, st, boffs);
-    case 3: return dispatch_encode_d<3>(d, data, z, out, offset, bits, st, boffs);
-    case 4: return dispatch_encode_d<4>(d, data, z, out, offset, bits, st, boffs);
+    case 1: return dispatch_encode_d<1>(d, data, z, out, offset, bits, st, boffs, at);
+    case 2: return dispatch_encode_d<2>(d, data, z, out, offset, bits, st, boffs, at);
+    case 3: return dispatch_encode_d<3>(d, data, z, out, offset, bits, st, boffs, at);
+    case 4: return dispatch_encode_d<4>(d, data, z, out, offset, bits, st, boffs, at);
   }
   return ZFP_HIP_EINVAL;
 }
@@ -368,6 +368,139 @@ extern "C" int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4],
   u64 bits = 0;
   rc = dispatch_decode(d, d_data, z, d_words, nwords, offset, boffs, &bits, (hipStream_t)hip_stream);
   if (rc)
+    return rc;
+  if (bits_out)
+    *bits_out = bits;
+  return ZFP_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// block codec for the compressed arrays: count whole blocks stored one after
+// another (block k at 4^dims * k values, x fastest).  As a field they are
+// an array of extents 4 x ... x 4*count with default strides, whose stream
+// order is the order of the blocks.
+// ---------------------------------------------------------------------------
+namespace {
+
+// slots -> bit offsets (slot * maxbits); flag[0] is set when a block would
+// end past the stream's nwords words
+__global__ void k_slot_offsets(const u64* __restrict__ slots, u64 n, u64 maxbits, u64 offset, u64 nwords,
+                               u64* __restrict__ at, u64* __restrict__ flag)
+{
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const u64 s = slots[i];
+  const u64 cap = nwords >= (1ull << 58) ? ~0ull : nwords * 64; // the stream's bits
+  // the slots that fit whole after `offset`
+  const u64 fit = cap >= offset ? (cap - offset) / maxbits : 0;
+  const bool ok = s < fit;
+  at[i] = ok ? s * maxbits : 0;
+  if (!ok)
+    atomicOr((unsigned long long*)flag, 1ull);
+}
+
+int block_field(unsigned dims, size_t count, size_t* n)
+{
+  if (dims < 1 || dims > 4 || count == 0 || count > (1ull << 29))
+    return record_error(ZFP_HIP_EINVAL, "block count or dimensionality out of range");
+  for (int i = 0; i < 4; i++)
+    n[i] = 0;
+  for (unsigned i = 0; i + 1 < dims; i++)
+    n[i] = 4;
+  n[dims - 1] = 4 * count;
+  return ZFP_HIP_OK;
+}
+
+// device bit offsets of the slots, checked against the stream (lock held)
+int slot_offsets(const u64* d_slots, size_t count, const Params& z, u64 offset, u64 nwords, hipStream_t st,
+                 const u64** at)
+{
+  if (!fixed_rate(z) || (z.maxbits & 63) || (offset & 63))
+    return record_error(ZFP_HIP_EINVAL, "blocks at chosen slots need fixed rate with word-aligned blocks");
+  void* p;
+  int rc = g_lens.get((count + 1) * sizeof(u64), &p);
+  if (rc)
+    return rc;
+  u64* flag = (u64*)p;
+  u64* off = flag + 1;
+  HIPCHK(hipMemsetAsync(flag, 0, sizeof(u64), st));
+  hipLaunchKernelGGL(k_slot_offsets, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, d_slots, (u64)count,
+                     (u64)z.maxbits, offset, nwords, off, flag);
+  HIPCHK(hipGetLastError());
+  u64 bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, flag, sizeof(u64), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (bad)
+    return record_error(ZFP_HIP_EINVAL, "a slot lies past the end of the stream");
+  *at = off;
+  return ZFP_HIP_OK;
+}
+
+} // namespace
+
+extern "C" int zfp_hip_encode_blocks(int type, unsigned dims, size_t count, const void* d_blocks,
+                                     const uint64_t* d_slots, uint minbits, uint maxbits, uint maxprec, int minexp,
+                                     uint64_t* d_words, uint64_t nwords, uint64_t offset, uint64_t* bits_out,
+                                     void* hip_stream)
+{
+  size_t n[4];
+  const ptrdiff_t s[4] = {0, 0, 0, 0};
+  int rc = block_field(dims, count, n);
+  if (rc)
+    return rc;
+  if (!d_slots)
+    return zfp_hip_encode(type, n, s, d_blocks, minbits, maxbits, maxprec, minexp, d_words, offset, bits_out,
+                          hip_stream);
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if ((rc = device_check()))
+    return rc;
+  Desc d;
+  if ((rc = make_desc(type, n, s, d)))
+    return rc;
+  if (minbits > maxbits || !maxprec || maxprec > 64 || !d_blocks || !d_words)
+    return record_error(ZFP_HIP_EINVAL, "invalid codec parameters or null pointer");
+  Params z{minbits, maxbits, maxprec, minexp};
+  const hipStream_t st = (hipStream_t)hip_stream;
+  const u64* at = nullptr;
+  if ((rc = slot_offsets(d_slots, count, z, offset, nwords, st, &at)))
+    return rc;
+  u64 bits = 0;
+  if ((rc = dispatch_encode(d, d_blocks, z, d_words, offset, &bits, st, nullptr, at)))
+    return rc;
+  if (bits_out)
+    *bits_out = bits;
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_decode_blocks(int type, unsigned dims, size_t count, void* d_blocks,
+                                     const uint64_t* d_slots, uint minbits, uint maxbits, uint maxprec, int minexp,
+                                     const uint64_t* d_words, uint64_t nwords, uint64_t offset, uint64_t* bits_out,
+                                     void* hip_stream)
+{
+  size_t n[4];
+  const ptrdiff_t s[4] = {0, 0, 0, 0};
+  int rc = block_field(dims, count, n);
+  if (rc)
+    return rc;
+  if (!d_slots)
+    return zfp_hip_decode(type, n, s, d_blocks, minbits, maxbits, maxprec, minexp, d_words, offset, bits_out,
+                          hip_stream);
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if ((rc = device_check()))
+    return rc;
+  Desc d;
+  if ((rc = make_desc(type, n, s, d)))
+    return rc;
+  if (minbits > maxbits || !maxprec || maxprec > 64 || !d_blocks || !d_words)
+    return record_error(ZFP_HIP_EINVAL, "invalid codec parameters or null pointer");
+  Params z{minbits, maxbits, maxprec, minexp};
+  const hipStream_t st = (hipStream_t)hip_stream;
+  const u64* at = nullptr;
+  if ((rc = slot_offsets(d_slots, count, z, offset, nwords, st, &at)))
+    return rc;
+  u64 bits = 0;
+  if ((rc = dispatch_decode(d, d_blocks, z, d_words, nwords, offset, at, &bits, st)))
     return rc;
   if (bits_out)
     *bits_out = bits;
