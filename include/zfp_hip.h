@@ -119,6 +119,23 @@ int zfp_hip_index_export(const void* stream_begin, uint64_t* offsets, size_t cou
 int zfp_hip_index_import(const void* stream_begin, const uint64_t* offsets, size_t count);
 void zfp_hip_index_clear(void);
 
+/* The same index in the reference's hybrid4 layout
+   (include/zfp/index.hpp:152-310, the index of its variable-rate compressed
+   arrays): per chunk of 4 blocks a record {uint32 hi; uint16 lo[4]} (12
+   bytes), block offset = (hi << 12) + lo[block % 4] bits from the first
+   block; ceil(nblocks / 4) records, the last padded with zero-length
+   blocks.  `start` is the bit position of the first block in the stream,
+   `end_bits` the offset just past the last block (hybrid4's range()).  The
+   conversions run on the host and need no device; they fail
+   (ZFP_HIP_EINVAL) where the reference's index throws: a block longer than
+   ZFP_MAX_BITS or an offset of 2^44 bits or more. */
+int zfp_hip_index_export_hybrid4(const void* stream_begin, void* records, size_t nrecords, uint64_t* start,
+                                 uint64_t* end_bits);
+int zfp_hip_index_import_hybrid4(const void* stream_begin, uint64_t start, const void* records, size_t nblocks,
+                                 uint64_t end_bits);
+int zfp_hip_hybrid4_from_offsets(const uint64_t* offsets, size_t nblocks, void* records);
+int zfp_hip_offsets_from_hybrid4(const void* records, size_t nblocks, uint64_t end_bits, uint64_t* offsets);
+
 /*
  * Batched codec: no counterpart in the reference, which compresses one field
  * per zfp_compress() call.  Many device-resident fields of mixed type

@@ -1,0 +1,111 @@
+"""Parallel decompression of variable-rate streams through a block index in
+the reference's hybrid4 layout (include/zfp/index.hpp:152-310).
+
+* A stream written by the reference library itself (serial zfp_compress,
+  oracle/_ref/libzfp_ref.so), with the hybrid4 index the reference's own
+  class builds from that stream's block sizes, is decoded on the GPU with
+  the index imported: the result equals serial decompression bit for bit.
+* A stream written here exports its index as hybrid4 records equal to the
+  reference's for the same block sizes; imported back after the library's
+  index registry is cleared, it decodes the same.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import zfp_amd as Z
+from zfptest import (ROOT, T_DOUBLE, T_FLOAT, api_compress, have_ref, oracle_compress, oracle_decompress,
+                     ref_api, smooth_field, stream_params_accuracy, stream_params_precision)
+
+pytestmark = pytest.mark.gpu
+REF4 = os.path.join(ROOT, "oracle", "_ref", "libhybrid4_ref.so")
+needs_ref = pytest.mark.skipif(not (have_ref() and os.path.exists(REF4)), reason="reference builds not present")
+U64P = ctypes.POINTER(ctypes.c_uint64)
+SZ4 = ctypes.c_size_t * 4
+PD4 = ctypes.c_ssize_t * 4
+
+
+def _ref_hybrid4(sizes):
+    lib = ctypes.CDLL(REF4)
+    lib.ref_hybrid4.argtypes = [U64P, ctypes.c_size_t, ctypes.c_void_p, U64P]
+    n = len(sizes)
+    s = np.asarray(sizes, dtype=np.uint64)
+    rec = np.zeros(((n + 3) // 4) * 12, dtype=np.uint8)
+    off = np.zeros(n + 1, dtype=np.uint64)
+    assert lib.ref_hybrid4(s.ctypes.data_as(U64P), n, rec.ctypes.data, off.ctypes.data_as(U64P)) == 0
+    return rec, off
+
+
+def _decode(ztype, n, p, d_words, out):
+    nn = SZ4(*(list(n) + [0] * (4 - len(n))))
+    bits = ctypes.c_uint64()
+    rc = Z.hip_lib().zfp_hip_decode(ztype, nn, PD4(0, 0, 0, 0), ctypes.c_void_p(out.data_ptr()), *p,
+                                    ctypes.c_void_p(d_words.data_ptr()), 0, ctypes.byref(bits), None)
+    return rc, bits.value
+
+
+CASES = [(T_DOUBLE, [40, 36, 28], stream_params_accuracy(1e-6)), (T_FLOAT, [257, 131], stream_params_precision(18)),
+         (T_DOUBLE, [20, 12, 16, 8], stream_params_precision(24))]
+
+
+@needs_ref
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_reference_stream_with_reference_hybrid4_index(k):
+    ztype, n, p = CASES[k]
+    a = smooth_field(n, ztype, seed=40 + k)
+    words, size = api_compress(ref_api(), a, 0, n, None, ztype, p)      # serial zfp, the reference library
+    worst, bits, per_block = oracle_compress(a, 0, n, None, ztype, p, block_bits=True)
+    rec, off = _ref_hybrid4([int(b) for b in per_block])
+    assert int(off[-1]) == bits
+    # the decoder's buffer contract: room for the worst-case stream
+    buf = np.zeros(worst.size, dtype=np.uint64)
+    buf[:words.size] = words
+    d_words = torch.from_numpy(buf.view(np.int64)).to("cuda:0")
+    Z.hip_lib().zfp_hip_index_clear()
+    rc = Z.hip_lib().zfp_hip_index_import_hybrid4(ctypes.c_void_p(d_words.data_ptr()), 0, rec.ctypes.data,
+                                                  len(per_block), int(off[-1]))
+    assert rc == 0
+    assert Z.hip_lib().zfp_hip_index_size(ctypes.c_void_p(d_words.data_ptr())) == len(per_block) + 1
+    ref = np.zeros_like(a)
+    oracle_decompress(words, ref, 0, n, None, ztype, p)
+    out = torch.zeros(a.size, dtype=torch.from_numpy(a).dtype, device="cuda:0")
+    rc, dbits = _decode(ztype, n, p, d_words, out)
+    assert rc == 0 and dbits == bits
+    assert np.array_equal(out.cpu().numpy().view(np.uint8), ref.view(np.uint8))
+    Z.hip_lib().zfp_hip_index_clear()
+
+
+@needs_ref
+def test_own_index_exported_as_hybrid4():
+    ztype, n, p = CASES[0]
+    a = smooth_field(n, ztype, seed=77)
+    exp_words, bits, per_block = oracle_compress(a, 0, n, None, ztype, p, block_bits=True)
+    d_words = torch.zeros(exp_words.size + 8, dtype=torch.int64, device="cuda:0")
+    d_a = torch.from_numpy(a).to("cuda:0")
+    nn = SZ4(*(list(n) + [0] * (4 - len(n))))
+    ebits = ctypes.c_uint64()
+    Z.hip_lib().zfp_hip_index_clear()
+    assert Z.hip_lib().zfp_hip_encode(ztype, nn, PD4(0, 0, 0, 0), ctypes.c_void_p(d_a.data_ptr()), *p,
+                                      ctypes.c_void_p(d_words.data_ptr()), 0, ctypes.byref(ebits), None) == 0
+    assert ebits.value == bits
+    nblocks = len(per_block)
+    rec = np.zeros(((nblocks + 3) // 4) * 12, dtype=np.uint8)
+    start, end = ctypes.c_uint64(), ctypes.c_uint64()
+    assert Z.hip_lib().zfp_hip_index_export_hybrid4(ctypes.c_void_p(d_words.data_ptr()), rec.ctypes.data,
+                                                    (nblocks + 3) // 4, ctypes.byref(start), ctypes.byref(end)) == 0
+    ref_rec, ref_off = _ref_hybrid4([int(b) for b in per_block])
+    assert start.value == 0 and end.value == int(ref_off[-1])
+    assert np.array_equal(rec, ref_rec)
+    Z.hip_lib().zfp_hip_index_clear()
+    assert Z.hip_lib().zfp_hip_index_import_hybrid4(ctypes.c_void_p(d_words.data_ptr()), 0, rec.ctypes.data,
+                                                    nblocks, end.value) == 0
+    ref = np.zeros_like(a)
+    oracle_decompress(exp_words, ref, 0, n, None, ztype, p)
+    out = torch.zeros_like(d_a)
+    rc, dbits = _decode(ztype, n, p, d_words, out)
+    assert rc == 0 and dbits == bits
+    assert np.array_equal(out.cpu().numpy().view(np.uint8), ref.view(np.uint8))
+    Z.hip_lib().zfp_hip_index_clear()

@@ -164,6 +164,13 @@ _HIP = {
                                              ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                              _P(ctypes.c_uint64), ctypes.c_void_p]),
     "zfp_hip_index_size": (ctypes.c_size_t, [ctypes.c_void_p]),
+    "zfp_hip_index_export_hybrid4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                    _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
+    "zfp_hip_index_import_hybrid4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                                    ctypes.c_size_t, ctypes.c_uint64]),
+    "zfp_hip_hybrid4_from_offsets": (ctypes.c_int, [_P(ctypes.c_uint64), ctypes.c_size_t, ctypes.c_void_p]),
+    "zfp_hip_offsets_from_hybrid4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64,
+                                                    _P(ctypes.c_uint64)]),
     "zfp_hip_index_export": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), ctypes.c_size_t]),
     "zfp_hip_index_import": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_uint64), ctypes.c_size_t]),
     "zfp_hip_index_clear": (None, []),

@@ -726,6 +726,94 @@ extern "C" int zfp_hip_index_import(const void* stream_begin, const uint64_t* of
   return ZFP_HIP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// the reference's hybrid4 block index (include/zfp/index.hpp:152-310): per
+// chunk of 4 blocks a 32-bit base (offset of the chunk's first block >> 12)
+// and four 16-bit offsets from base << 12; the last chunk is padded with
+// zero-length blocks; blocks of at most ZFP_MAX_BITS bits, offsets below
+// 2^44.  Offsets count from the first block.
+// ---------------------------------------------------------------------------
+namespace {
+struct Hybrid4Record {
+  uint32_t hi;
+  uint16_t lo[4];
+};
+static_assert(sizeof(Hybrid4Record) == 12, "hybrid4 record layout");
+} // namespace
+
+extern "C" int zfp_hip_hybrid4_from_offsets(const uint64_t* offsets, size_t nblocks, void* records)
+{
+  if (!offsets || !records || !nblocks)
+    return record_error(ZFP_HIP_EINVAL, "invalid index");
+  const u64 base0 = offsets[0];
+  Hybrid4Record* r = (Hybrid4Record*)records;
+  for (size_t b = 0; b < nblocks; b++)
+    if (offsets[b + 1] < offsets[b] || offsets[b + 1] - offsets[b] > ZFP_MAX_BITS)
+      return record_error(ZFP_HIP_EINVAL, "block size out of range for a hybrid4 index");
+  for (size_t c = 0; c < (nblocks + 3) / 4; c++) {
+    const u64 ptr = offsets[4 * c] - base0;
+    if (ptr >> 44)
+      return record_error(ZFP_HIP_EINVAL, "block offset too large for a hybrid4 index");
+    r[c].hi = (uint32_t)(ptr >> 12);
+    const u64 base = (u64)r[c].hi << 12;
+    for (int k = 0; k < 4; k++) {
+      const size_t b = 4 * c + k;
+      r[c].lo[k] = (uint16_t)(offsets[b < nblocks ? b : nblocks] - base0 - base);
+    }
+  }
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_offsets_from_hybrid4(const void* records, size_t nblocks, uint64_t end_bits,
+                                            uint64_t* offsets)
+{
+  if (!offsets || !records || !nblocks)
+    return record_error(ZFP_HIP_EINVAL, "invalid index");
+  const Hybrid4Record* r = (const Hybrid4Record*)records;
+  for (size_t b = 0; b < nblocks; b++)
+    offsets[b] = ((u64)r[b / 4].hi << 12) + r[b / 4].lo[b % 4];
+  offsets[nblocks] = end_bits;
+  for (size_t b = 0; b < nblocks; b++)
+    if (offsets[b + 1] < offsets[b])
+      return record_error(ZFP_HIP_EINVAL, "hybrid4 offsets decrease");
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_index_export_hybrid4(const void* stream_begin, void* records, size_t nrecords,
+                                            uint64_t* start, uint64_t* end_bits)
+{
+  const size_t count = zfp_hip_index_size(stream_begin);
+  if (!count)
+    return record_error(ZFP_HIP_ENOINDEX, "no block index for this stream");
+  if (!records || nrecords < (count - 1 + 3) / 4)
+    return record_error(ZFP_HIP_EINVAL, "record array too small");
+  std::vector<uint64_t> off(count);
+  int rc = zfp_hip_index_export(stream_begin, off.data(), count);
+  if (rc)
+    return rc;
+  if ((rc = zfp_hip_hybrid4_from_offsets(off.data(), count - 1, records)))
+    return rc;
+  if (start)
+    *start = off[0];
+  if (end_bits)
+    *end_bits = off[count - 1] - off[0];
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_index_import_hybrid4(const void* stream_begin, uint64_t start, const void* records,
+                                            size_t nblocks, uint64_t end_bits)
+{
+  if (!nblocks || nblocks > 0xffffffffull)
+    return record_error(ZFP_HIP_EINVAL, "invalid index");
+  std::vector<uint64_t> off(nblocks + 1);
+  int rc = zfp_hip_offsets_from_hybrid4(records, nblocks, end_bits, off.data());
+  if (rc)
+    return rc;
+  for (auto& o : off)
+    o += start;
+  return zfp_hip_index_import(stream_begin, off.data(), off.size());
+}
+
 extern "C" void zfp_hip_index_clear(void)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
