@@ -136,6 +136,18 @@ int zfp_hip_index_import_hybrid4(const void* stream_begin, uint64_t start, const
 int zfp_hip_hybrid4_from_offsets(const uint64_t* offsets, size_t nblocks, void* records);
 int zfp_hip_offsets_from_hybrid4(const void* records, size_t nblocks, uint64_t end_bits, uint64_t* offsets);
 
+/* The index in the reference's hybrid8<dims> layout (include/zfp/index.hpp,
+   class hybrid8): two 64-bit words per chunk of 8 blocks, 2 * ceil(nblocks
+   / 8) words, as zfp::index::hybrid8<dims> stores them; blocks shorter than
+   2^(8 + 2 (dims - 1)) bits.  start and end_bits as for hybrid4. */
+int zfp_hip_index_export_hybrid8(const void* stream_begin, unsigned dims, uint64_t* words, size_t nwords,
+                                 uint64_t* start, uint64_t* end_bits);
+int zfp_hip_index_import_hybrid8(const void* stream_begin, unsigned dims, uint64_t start, const uint64_t* words,
+                                 size_t nblocks, uint64_t end_bits);
+int zfp_hip_hybrid8_from_offsets(unsigned dims, const uint64_t* offsets, size_t nblocks, uint64_t* words);
+int zfp_hip_offsets_from_hybrid8(unsigned dims, const uint64_t* words, size_t nblocks, uint64_t end_bits,
+                                 uint64_t* offsets);
+
 /*
  * Batched codec: no counterpart in the reference, which compresses one field
  * per zfp_compress() call.  Many device-resident fields of mixed type

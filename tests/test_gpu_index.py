@@ -1,10 +1,11 @@
 """Parallel decompression of variable-rate streams through a block index in
-the reference's hybrid4 layout (include/zfp/index.hpp:152-310).
+the reference's hybrid4 and hybrid8 layouts (include/zfp/index.hpp).
 
 * A stream written by the reference library itself (serial zfp_compress,
-  oracle/_ref/libzfp_ref.so), with the hybrid4 index the reference's own
-  class builds from that stream's block sizes, is decoded on the GPU with
-  the index imported: the result equals serial decompression bit for bit.
+  oracle/_ref/libzfp_ref.so), with the hybrid4 or hybrid8 index the
+  reference's own classes build from that stream's block sizes, is decoded
+  on the GPU with the index imported: the result equals serial
+  decompression bit for bit.
 * A stream written here exports its index as hybrid4 records equal to the
   reference's for the same block sizes; imported back after the library's
   index registry is cleared, it decodes the same.
@@ -39,6 +40,17 @@ def _ref_hybrid4(sizes):
     return rec, off
 
 
+def _ref_hybrid8(dims, sizes):
+    lib = ctypes.CDLL(REF4)
+    lib.ref_hybrid8.argtypes = [ctypes.c_uint, U64P, ctypes.c_size_t, U64P, U64P]
+    n = len(sizes)
+    s = np.asarray(sizes, dtype=np.uint64)
+    words = np.zeros(2 * ((n + 7) // 8), dtype=np.uint64)
+    off = np.zeros(n + 1, dtype=np.uint64)
+    assert lib.ref_hybrid8(dims, s.ctypes.data_as(U64P), n, words.ctypes.data_as(U64P), off.ctypes.data_as(U64P)) == 0
+    return words, off
+
+
 def _decode(ztype, n, p, d_words, out):
     nn = SZ4(*(list(n) + [0] * (4 - len(n))))
     bits = ctypes.c_uint64()
@@ -52,21 +64,27 @@ CASES = [(T_DOUBLE, [40, 36, 28], stream_params_accuracy(1e-6)), (T_FLOAT, [257,
 
 
 @needs_ref
+@pytest.mark.parametrize("layout", ["hybrid4", "hybrid8"])
 @pytest.mark.parametrize("k", range(len(CASES)))
-def test_reference_stream_with_reference_hybrid4_index(k):
+def test_reference_stream_with_reference_index(k, layout):
     ztype, n, p = CASES[k]
     a = smooth_field(n, ztype, seed=40 + k)
     words, size = api_compress(ref_api(), a, 0, n, None, ztype, p)      # serial zfp, the reference library
     worst, bits, per_block = oracle_compress(a, 0, n, None, ztype, p, block_bits=True)
-    rec, off = _ref_hybrid4([int(b) for b in per_block])
+    sizes = [int(b) for b in per_block]
+    rec, off = _ref_hybrid4(sizes) if layout == "hybrid4" else _ref_hybrid8(len(n), sizes)
     assert int(off[-1]) == bits
     # the decoder's buffer contract: room for the worst-case stream
     buf = np.zeros(worst.size, dtype=np.uint64)
     buf[:words.size] = words
     d_words = torch.from_numpy(buf.view(np.int64)).to("cuda:0")
     Z.hip_lib().zfp_hip_index_clear()
-    rc = Z.hip_lib().zfp_hip_index_import_hybrid4(ctypes.c_void_p(d_words.data_ptr()), 0, rec.ctypes.data,
-                                                  len(per_block), int(off[-1]))
+    if layout == "hybrid4":
+        rc = Z.hip_lib().zfp_hip_index_import_hybrid4(ctypes.c_void_p(d_words.data_ptr()), 0, rec.ctypes.data,
+                                                      len(per_block), int(off[-1]))
+    else:
+        rc = Z.hip_lib().zfp_hip_index_import_hybrid8(ctypes.c_void_p(d_words.data_ptr()), len(n), 0,
+                                                      rec.ctypes.data_as(U64P), len(per_block), int(off[-1]))
     assert rc == 0
     assert Z.hip_lib().zfp_hip_index_size(ctypes.c_void_p(d_words.data_ptr())) == len(per_block) + 1
     ref = np.zeros_like(a)

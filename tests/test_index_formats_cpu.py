@@ -1,9 +1,9 @@
-"""The hybrid4 block-index layout of the reference's variable-rate
-compressed arrays (include/zfp/index.hpp:152-310), as exported and imported
-by include/zfp_hip.h, against the reference's own hybrid4 class compiled
-from its header (oracle/_ref/libhybrid4_ref.so): the same 12-byte records
-byte for byte, the same offsets back, the same refusals.  Host-only
-conversions: no GPU needed."""
+"""The hybrid4 and hybrid8<dims> block-index layouts of the reference's
+variable-rate compressed arrays (include/zfp/index.hpp), as exported and
+imported by include/zfp_hip.h, against the reference's own index classes
+compiled from its header (oracle/_ref/libhybrid4_ref.so): the same records
+and words byte for byte, the same offsets back, the same refusals.
+Host-only conversions: no GPU needed."""
 import ctypes
 import os
 
@@ -90,3 +90,51 @@ def test_decreasing_records_are_refused():
     rec[4:12].view(np.uint16)[:] = [0, 50, 20, 60]
     out = np.zeros(5, dtype=np.uint64)
     assert Z.hip_lib().zfp_hip_offsets_from_hybrid4(rec.ctypes.data, 4, 80, out.ctypes.data_as(U64P)) == Z.ZFP_HIP_EINVAL
+
+
+# ---------------------------------------------------------------------------
+# hybrid8<dims>
+# ---------------------------------------------------------------------------
+
+def _ref8(dims, sizes):
+    lib = ctypes.CDLL(REF)
+    lib.ref_hybrid8.argtypes = [ctypes.c_uint, U64P, ctypes.c_size_t, U64P, U64P]
+    n = len(sizes)
+    s = np.asarray(sizes, dtype=np.uint64)
+    words = np.zeros(2 * ((n + 7) // 8), dtype=np.uint64)
+    off = np.zeros(n + 1, dtype=np.uint64)
+    rc = lib.ref_hybrid8(dims, s.ctypes.data_as(U64P), n, words.ctypes.data_as(U64P), off.ctypes.data_as(U64P))
+    return rc, words, off
+
+
+@needs_ref
+@pytest.mark.parametrize("dims", [1, 2, 3, 4])
+@pytest.mark.parametrize("shape", ["one", "ragged", "random", "max"])
+def test_hybrid8_words_equal_the_reference(dims, shape):
+    limit = 1 << (8 + 2 * (dims - 1))     # block sizes below this
+    rng = np.random.default_rng(dims)
+    sizes = {"one": [limit - 1], "ragged": [0, 3, limit - 1, 7, 0, 0, 255, 256 % limit, 1, 2, 9],
+             "random": list(rng.integers(0, limit, size=1003)), "max": [limit - 1] * 17}[shape]
+    rc, ref_words, off = _ref8(dims, sizes)
+    assert rc == 0
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(np.asarray(sizes, dtype=np.uint64))]))
+    words = np.zeros_like(ref_words)
+    assert Z.hip_lib().zfp_hip_hybrid8_from_offsets(dims, off.ctypes.data_as(U64P), len(sizes),
+                                                    words.ctypes.data_as(U64P)) == 0
+    assert np.array_equal(words, ref_words)
+    back = np.zeros_like(off)
+    assert Z.hip_lib().zfp_hip_offsets_from_hybrid8(dims, ref_words.ctypes.data_as(U64P), len(sizes), int(off[-1]),
+                                                    back.ctypes.data_as(U64P)) == 0
+    assert np.array_equal(back, off)
+
+
+@needs_ref
+@pytest.mark.parametrize("dims", [1, 2, 3, 4])
+def test_hybrid8_block_too_long_is_refused_by_both(dims):
+    sizes = [5, 1 << (8 + 2 * (dims - 1)), 5]
+    rc, _, _ = _ref8(dims, sizes)
+    assert rc == 1
+    off = np.concatenate([[0], np.cumsum(np.asarray(sizes, dtype=np.uint64))])
+    words = np.zeros(2, dtype=np.uint64)
+    assert Z.hip_lib().zfp_hip_hybrid8_from_offsets(dims, off.ctypes.data_as(U64P), 3,
+                                                    words.ctypes.data_as(U64P)) == Z.ZFP_HIP_EINVAL

@@ -168,6 +168,14 @@ _HIP = {
                                                     _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
     "zfp_hip_index_import_hybrid4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
                                                     ctypes.c_size_t, ctypes.c_uint64]),
+    "zfp_hip_index_export_hybrid8": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, _P(ctypes.c_uint64), ctypes.c_size_t,
+                                                    _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),
+    "zfp_hip_index_import_hybrid8": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint64,
+                                                    _P(ctypes.c_uint64), ctypes.c_size_t, ctypes.c_uint64]),
+    "zfp_hip_hybrid8_from_offsets": (ctypes.c_int, [ctypes.c_uint, _P(ctypes.c_uint64), ctypes.c_size_t,
+                                                    _P(ctypes.c_uint64)]),
+    "zfp_hip_offsets_from_hybrid8": (ctypes.c_int, [ctypes.c_uint, _P(ctypes.c_uint64), ctypes.c_size_t,
+                                                    ctypes.c_uint64, _P(ctypes.c_uint64)]),
     "zfp_hip_hybrid4_from_offsets": (ctypes.c_int, [_P(ctypes.c_uint64), ctypes.c_size_t, ctypes.c_void_p]),
     "zfp_hip_offsets_from_hybrid4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64,
                                                     _P(ctypes.c_uint64)]),

@@ -814,6 +814,106 @@ extern "C" int zfp_hip_index_import_hybrid4(const void* stream_begin, uint64_t s
   return zfp_hip_index_import(stream_begin, off.data(), off.size());
 }
 
+// ---------------------------------------------------------------------------
+// the reference's hybrid8<dims> block index (include/zfp/index.hpp:312-...):
+// per chunk of 8 blocks two words.  A block size is split into its low 8
+// bits and hb = 2 (dims - 1) high bits; so is the chunk's first offset.
+// Word 0 holds the offset's high part above 7 hb-bit fields that hold the
+// high bits of sizes 0..6; word 1 holds the offset's low 8 bits above seven
+// 8-bit fields with the low bits of sizes 0..6.  Block k's offset is the
+// chunk's offset plus sizes 0..k-1, i.e. the sums of the top k+1 fields of
+// each word.  Sizes must be below 2^(8 + hb), the chunk offset's high part
+// must fit its field.  Offsets count from the first block.
+// ---------------------------------------------------------------------------
+extern "C" int zfp_hip_hybrid8_from_offsets(unsigned dims, const uint64_t* offsets, size_t nblocks, uint64_t* words)
+{
+  if (!offsets || !words || !nblocks || dims < 1 || dims > 4)
+    return record_error(ZFP_HIP_EINVAL, "invalid index");
+  const unsigned hb = 2 * (dims - 1);
+  const u64 b0 = offsets[0];
+  for (size_t b = 0; b < nblocks; b++)
+    if (offsets[b + 1] < offsets[b] || ((offsets[b + 1] - offsets[b]) >> (hb + 8)))
+      return record_error(ZFP_HIP_EINVAL, "block size out of range for a hybrid8 index");
+  for (size_t c = 0; c < (nblocks + 7) / 8; c++) {
+    const u64 ptr = offsets[8 * c] - b0;
+    const u64 h = ptr >> 8;
+    u64 hi = h << (7 * hb);
+    if ((hi >> (7 * hb)) != h)
+      return record_error(ZFP_HIP_EINVAL, "block offset too large for a hybrid8 index");
+    u64 lo = (ptr & 0xff) << 56;
+    for (unsigned k = 0; k < 7; k++) {
+      const size_t b = 8 * c + k;
+      const u64 size = b < nblocks ? offsets[b + 1] - offsets[b] : 0;
+      hi += (size >> 8) << ((6 - k) * hb);
+      lo += (size & 0xff) << ((6 - k) * 8);
+    }
+    words[2 * c] = hi;
+    words[2 * c + 1] = lo;
+  }
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_offsets_from_hybrid8(unsigned dims, const uint64_t* words, size_t nblocks, uint64_t end_bits,
+                                            uint64_t* offsets)
+{
+  if (!offsets || !words || !nblocks || dims < 1 || dims > 4)
+    return record_error(ZFP_HIP_EINVAL, "invalid index");
+  const unsigned hb = 2 * (dims - 1);
+  const u64 hmask = hb ? (1ull << hb) - 1 : 0;
+  for (size_t b = 0; b < nblocks; b++) {
+    const size_t c = b / 8;
+    const unsigned k = (unsigned)(b % 8);
+    const u64 hi = words[2 * c], lo = words[2 * c + 1];
+    const u64 base = hb ? hi >> (8 * hb) : hi; // the offset's high part above field 7
+    u64 hs = 0, ls = 0;
+    for (unsigned f = 7 - k; f <= 7; f++) {
+      hs += hb ? (hi >> (f * hb)) & hmask : 0;
+      ls += (lo >> (8 * f)) & 0xff;
+    }
+    offsets[b] = (((base << hb) + hs) << 8) + ls;
+  }
+  offsets[nblocks] = end_bits;
+  for (size_t b = 0; b < nblocks; b++)
+    if (offsets[b + 1] < offsets[b])
+      return record_error(ZFP_HIP_EINVAL, "hybrid8 offsets decrease");
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_index_export_hybrid8(const void* stream_begin, unsigned dims, uint64_t* words,
+                                            size_t nwords, uint64_t* start, uint64_t* end_bits)
+{
+  const size_t count = zfp_hip_index_size(stream_begin);
+  if (!count)
+    return record_error(ZFP_HIP_ENOINDEX, "no block index for this stream");
+  if (!words || nwords < 2 * ((count - 1 + 7) / 8))
+    return record_error(ZFP_HIP_EINVAL, "word array too small");
+  std::vector<uint64_t> off(count);
+  int rc = zfp_hip_index_export(stream_begin, off.data(), count);
+  if (rc)
+    return rc;
+  if ((rc = zfp_hip_hybrid8_from_offsets(dims, off.data(), count - 1, words)))
+    return rc;
+  if (start)
+    *start = off[0];
+  if (end_bits)
+    *end_bits = off[count - 1] - off[0];
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_index_import_hybrid8(const void* stream_begin, unsigned dims, uint64_t start,
+                                            const uint64_t* words, size_t nblocks, uint64_t end_bits)
+{
+  if (!nblocks || nblocks > 0xffffffffull)
+    return record_error(ZFP_HIP_EINVAL, "invalid index");
+  std::vector<uint64_t> off(nblocks + 1);
+  int rc = zfp_hip_offsets_from_hybrid8(dims, words, nblocks, end_bits, off.data());
+  if (rc)
+    return rc;
+  for (auto& o : off)
+    o += start;
+  return zfp_hip_index_import(stream_begin, off.data(), off.size());
+}
+
 extern "C" void zfp_hip_index_clear(void)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
