@@ -27,8 +27,9 @@
 namespace zfphip {
 
 // ENC_FIXED_PLAIN: fixed rate with every block range word-aligned, so each
-// lane owns whole words and stores them without atomics
-enum { ENC_FIXED = 0, ENC_OFFSETS = 1, ENC_LENGTH = 2, ENC_FIXED_PLAIN = 3 };
+// lane owns whole words and stores them without atomics; ENC_FIXED_AT: the
+// same with block b placed at bit offset + boffs[b] (chosen slots)
+enum { ENC_FIXED = 0, ENC_OFFSETS = 1, ENC_LENGTH = 2, ENC_FIXED_PLAIN = 3, ENC_FIXED_AT = 4 };
 
 // the block codec of the stream's mode: lossy, or reversible (REV)
 template <typename S, int D, bool REV, typename W>
@@ -70,15 +71,18 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
   }
   u64 start, end;
   if (MODE == ENC_FIXED || MODE == ENC_FIXED_PLAIN) {
-    // block b's place: b * maxbits, or boffs[b] when the caller chose it
-    start = offset + (boffs ? boffs[b] : (u64)b * z.maxbits);
+    start = offset + (u64)b * z.maxbits;
+    end = start + z.maxbits;
+  }
+  else if (MODE == ENC_FIXED_AT) {
+    start = offset + boffs[b];
     end = start + z.maxbits;
   }
   else {
     start = offset + boffs[b];
     end = offset + boffs[b + 1];
   }
-  BitWriter<MODE == ENC_FIXED_PLAIN> w;
+  BitWriter<MODE == ENC_FIXED_PLAIN || MODE == ENC_FIXED_AT> w;
   w.begin(out, start, end);
   encode_any<S, D, REV>(w, data, g, bc, z);
   w.finish();
@@ -335,7 +339,7 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
   if (at) {
     if ((offset & 63) || (z.maxbits & 63))
       return record_error(ZFP_HIP_EINVAL, "blocks at chosen places need word-aligned blocks");
-    hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+    hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_AT>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                        d.nblocks, z, out, offset, at, (u64*)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
