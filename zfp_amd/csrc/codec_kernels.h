@@ -9,7 +9,12 @@
 //                       embedded coder, writing the block's bits at its offset
 //                       (fixed rate: off + b*maxbits; otherwise from a scan)
 //                       or, in the length pass, only counting them
-//   k_decode<S,D>       the inverse, reading each block from its offset
+//   k_decode<S,D>       the inverse, reading each block from its offset;
+//                       for 3D doubles at fixed rates up to 16 bits/value a
+//                       first pass over planes 63..32 (HI_ONLY) lists the
+//                       blocks that k_decode_list then decodes in full
+//   k_encode_waves<S,D> variable-length write pass whose waves assemble
+//                       their words in LDS (lanes in stream order)
 //   k_parse<S,D>        one-lane sequential parse of block lengths for
 //                       variable-rate streams that arrive without an index
 //   k_prepare           zeroes the words that blocks will OR into
