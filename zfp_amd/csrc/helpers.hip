@@ -2,6 +2,7 @@
 // stream words that blocks OR into, and the exclusive prefix sum that turns
 // per-block lengths into stream offsets.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_scan.hpp>
 
 #include "backend.h"
 
@@ -95,13 +96,19 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_add(u64* out, u64 n, const u64
 
 u64 scan_tmp_words(u64 n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
+// one pass over the data (rocPRIM's decoupled look-back scan) with its
+// temporary storage from the grow-only scratch; exclusive_scan_tmp below is
+// the allocation-free variant the batch graphs capture
 int exclusive_scan(const u64* in, u64* out, u64 n, hipStream_t st)
 {
+  size_t bytes = 0;
+  HIPCHK(rocprim::exclusive_scan(nullptr, bytes, in, out, (u64)0, (size_t)n, rocprim::plus<u64>(), st));
   void* tp;
-  int rc = g_scan_tmp.get(scan_tmp_words(n) * sizeof(u64), &tp);
+  int rc = g_scan_tmp.get(bytes ? bytes : 8, &tp);
   if (rc)
     return rc;
-  return exclusive_scan_tmp(in, out, n, (u64*)tp, st);
+  HIPCHK(rocprim::exclusive_scan(tp, bytes, in, out, (u64)0, (size_t)n, rocprim::plus<u64>(), st));
+  return ZFP_HIP_OK;
 }
 
 int exclusive_scan_tmp(const u64* in, u64* out, u64 n, u64* sums, hipStream_t st)
