@@ -127,3 +127,39 @@ def test_own_index_exported_as_hybrid4():
     assert rc == 0 and dbits == bits
     assert np.array_equal(out.cpu().numpy().view(np.uint8), ref.view(np.uint8))
     Z.hip_lib().zfp_hip_index_clear()
+
+
+def test_repeated_encodes_into_one_buffer():
+    """Variable-rate fields of different sizes compressed one after another
+    into one stream buffer: each time the recorded index holds that field's
+    offsets (the registry hands a replaced index's buffer to the next encode)
+    and decoding through it equals serial decompression."""
+    ztype, p = T_DOUBLE, stream_params_accuracy(1e-5)
+    shapes = [[36, 28, 20], [9, 10, 11], [44, 40, 24], [9, 10, 11], [36, 28, 20]]
+    worst = max(oracle_compress(smooth_field(n, ztype, seed=1), 0, n, None, ztype, p)[0].size for n in shapes)
+    d_words = torch.zeros(worst + 8, dtype=torch.int64, device="cuda:0")
+    Z.hip_lib().zfp_hip_index_clear()
+    for k, n in enumerate(shapes):
+        a = smooth_field(n, ztype, seed=90 + k)
+        exp_words, bits, per_block = oracle_compress(a, 0, n, None, ztype, p, block_bits=True)
+        d_a = torch.from_numpy(a).to("cuda:0")
+        nn = SZ4(*(list(n) + [0] * (4 - len(n))))
+        ebits = ctypes.c_uint64()
+        assert Z.hip_lib().zfp_hip_encode(ztype, nn, PD4(0, 0, 0, 0), ctypes.c_void_p(d_a.data_ptr()), *p,
+                                          ctypes.c_void_p(d_words.data_ptr()), 0, ctypes.byref(ebits), None) == 0
+        assert ebits.value == bits
+        got = d_words.cpu().numpy().view(np.uint64)[:(bits + 63) // 64]
+        assert np.array_equal(got, exp_words[:got.size])
+        nb = len(per_block)
+        assert Z.hip_lib().zfp_hip_index_size(ctypes.c_void_p(d_words.data_ptr())) == nb + 1
+        off = np.zeros(nb + 1, dtype=np.uint64)
+        assert Z.hip_lib().zfp_hip_index_export(ctypes.c_void_p(d_words.data_ptr()), off.ctypes.data_as(U64P),
+                                                nb + 1) == 0
+        assert np.array_equal(off, np.concatenate([[0], np.cumsum(per_block, dtype=np.uint64)]))
+        ref = np.zeros_like(a)
+        oracle_decompress(exp_words, ref, 0, n, None, ztype, p)
+        out = torch.zeros_like(d_a)
+        rc, dbits = _decode(ztype, n, p, d_words, out)
+        assert rc == 0 and dbits == bits, f"shape {n}"
+        assert np.array_equal(out.cpu().numpy().view(np.uint8), ref.view(np.uint8)), f"shape {n}"
+    Z.hip_lib().zfp_hip_index_clear()

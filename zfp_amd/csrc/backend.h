@@ -88,10 +88,17 @@ struct Scratch {
 // entry points run under one library lock, so one set suffices
 extern Scratch g_lens, g_scan_tmp;
 
+// device buffer for a stream's block index of at least `bytes`: the spare
+// buffer of the index registry when it is large enough, else a new
+// allocation.  The caller owns it: it goes to the registry or back through
+// index_release.
+int index_buffer(u64 bytes, u64** p);
+void index_release(u64* p, u64 bytes);
+
 // launch the codec for one field; defined in codec_kernels.h and instantiated
 // for D = 1..4 in inst_d<D>.hip.  On variable-rate encodes, *boffs (when
-// non-null) receives a device copy of the nblocks+1 block offsets that the
-// caller must hipFree.
+// non-null) receives the nblocks+1 block offsets in a buffer from
+// index_buffer, which the caller owns.
 template <int D>
 int dispatch_encode_d(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
                       u64* bits, hipStream_t st, u64** boffs, const u64* at = nullptr);

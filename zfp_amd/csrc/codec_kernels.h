@@ -398,6 +398,22 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
     return rc;
   u64* lens = (u64*)lp;
   u64* offs = lens + nb1;
+  // the caller's copy of the offsets: the scan writes it directly
+  struct Held {
+    u64* p = nullptr;
+    u64 bytes = 0;
+    ~Held()
+    {
+      if (p)
+        index_release(p, bytes);
+    }
+  } held;
+  if (boffs_out) {
+    *boffs_out = nullptr;
+    held.bytes = nb1 * sizeof(u64);
+    if (index_buffer(held.bytes, &held.p) == ZFP_HIP_OK)
+      offs = held.p;
+  }
   HIPCHK(hipMemsetAsync(lens + d.nblocks, 0, sizeof(u64), st));
   hipLaunchKernelGGL((k_encode<S, D, ENC_LENGTH, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                      d.nblocks, z, out, offset, (const u64*)nullptr, lens);
@@ -432,16 +448,8 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
   HIPCHK(hipStreamSynchronize(st));
   *bits_out = total;
   if (boffs_out) {
-    // hand a private copy of the offsets to the caller (index registry)
-    u64* keep = nullptr;
-    if (hipMalloc(&keep, nb1 * sizeof(u64)) != hipSuccess) {
-      (void)hipGetLastError();
-      *boffs_out = nullptr;
-      return ZFP_HIP_OK;
-    }
-    HIPCHK(hipMemcpyAsync(keep, offs, nb1 * sizeof(u64), hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-    *boffs_out = keep;
+    *boffs_out = held.p;
+    held.p = nullptr;
   }
   return ZFP_HIP_OK;
 }

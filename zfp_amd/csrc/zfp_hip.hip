@@ -136,11 +136,17 @@ struct Index {
 };
 std::map<const void*, Index> g_index;
 
+// one index buffer that left the registry, kept for the next variable-rate
+// encode: compressing again and again into one stream buffer then allocates
+// no index after its second call
+u64* g_spare = nullptr;
+u64 g_spare_bytes = 0;
+
 void index_put(const void* key, const Index& ix)
 {
   auto it = g_index.find(key);
   if (it != g_index.end() && it->second.d_offsets)
-    (void)hipFree(it->second.d_offsets);
+    index_release(it->second.d_offsets, ((u64)it->second.nblocks + 1) * sizeof(u64));
   g_index[key] = ix;
 }
 
@@ -168,6 +174,35 @@ const Index* index_find(const void* key, u64 start, const Desc& d, const size_t*
 int g_dev_state = -1; // -1 unknown, 0 usable, else an error code
 
 } // namespace
+
+int index_buffer(u64 bytes, u64** p)
+{
+  if (g_spare && g_spare_bytes >= bytes) {
+    *p = g_spare;
+    g_spare = nullptr;
+    g_spare_bytes = 0;
+    return ZFP_HIP_OK;
+  }
+  if (hipMalloc((void**)p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return ZFP_HIP_ENOMEM;
+  }
+  return ZFP_HIP_OK;
+}
+
+// keeps the larger of the released buffer and the spare
+void index_release(u64* p, u64 bytes)
+{
+  if (g_spare && g_spare_bytes >= bytes) {
+    (void)hipFree(p);
+    return;
+  }
+  if (g_spare)
+    (void)hipFree(g_spare);
+  g_spare = p;
+  g_spare_bytes = bytes;
+}
 
 // called with the library lock held; also clears HIP error state left by
 // other users of the runtime in this thread, so that the checks after our
@@ -571,7 +606,7 @@ extern "C" size_t zfp_hip_compress(zfp_stream* zs, const zfp_field* field)
   const u64 words = (lead + bits + 63) / 64;
   if (!dev_stream && (e = hipMemcpy(bs->ptr, d_out, words * sizeof(u64), hipMemcpyDeviceToHost)) != hipSuccess) {
     if (boffs)
-      (void)hipFree(boffs);
+      index_release(boffs, ((u64)d.nblocks + 1) * sizeof(u64));
     return hip_failed(e, "copy of the stream to the host");
   }
   if (boffs) {
@@ -921,4 +956,8 @@ extern "C" void zfp_hip_index_clear(void)
     if (kv.second.d_offsets)
       (void)hipFree(kv.second.d_offsets);
   g_index.clear();
+  if (g_spare)
+    (void)hipFree(g_spare);
+  g_spare = nullptr;
+  g_spare_bytes = 0;
 }
