@@ -54,6 +54,8 @@ bool is_device_ptr(const void* p);
 
 // zero words [w0, w1) of a stream, keeping the low `keep` bits of word w0
 __global__ void k_prepare(u64* out, u64 w0, u64 w1, u32 keep);
+// the same for words [offset/64, (offset + *bits + 63)/64), *bits on the device
+__global__ void k_prepare_dev(u64* out, u64 offset, const u64* bits);
 // exclusive prefix sum of n u64 values (in and out may not alias)
 int exclusive_scan(const u64* in, u64* out, u64 n, hipStream_t st);
 // the same with a caller-owned temporary of scan_tmp_words(n) words; it

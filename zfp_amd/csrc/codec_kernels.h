@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <tuple>
 
 #include "backend.h"
 #include "codec_block.h"
@@ -421,22 +423,42 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
   rc = exclusive_scan(lens, offs, nb1, st);
   if (rc)
     return rc;
-  u64 total = 0;
-  HIPCHK(hipMemcpyAsync(&total, offs + d.nblocks, sizeof(u64), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  u64 w0 = offset >> 6, w1 = (offset + total + 63) >> 6;
-  if (w1 > w0)
-    hipLaunchKernelGGL(k_prepare, dim3(grid_for(w1 - w0, 256)), dim3(256), 0, st, out, w0, w1,
-                       (u32)(offset & 63));
   bool stream_order = true; // lane t holds stream block t
   for (int i = 0; i < d.dims; i++)
     stream_order = stream_order && d.g.lax[i] == (u32)i;
+  // a wave's LDS area in the write pass: twice the words 64 average blocks
+  // take, at least 64 and at most 2048 (16 KiB).  A wave whose range is
+  // longer writes directly, so the size sets the speed, never the stream.
+  // It is taken from the last encode with the same codec parameters; only
+  // the first such encode waits for the stream length on the host.
+  auto wcap_for = [&](u64 bits) {
+    const u64 avg_words = bits / (d.nblocks ? d.nblocks : 1) + 1;
+    return (u32)std::min<u64>(2048, std::max<u64>(64, 2 * avg_words + 4));
+  };
+  static std::map<std::tuple<u32, u32, u32, int>, u32> last_wcap;
+  const auto key = std::make_tuple(z.minbits, z.maxbits, z.maxprec, z.minexp);
+  u32 wcap = 0;
   if (stream_order) {
-    // a wave's LDS area: twice the words 64 average blocks take, at least
-    // 64 and at most 2048 (16 KiB); a wave whose range is longer writes
-    // directly
-    const u64 avg_words = (total / (d.nblocks ? d.nblocks : 1) + 1) * 64 / 64;
-    const u32 wcap = (u32)std::min<u64>(2048, std::max<u64>(64, 2 * avg_words + 4));
+    auto it = last_wcap.find(key);
+    if (it != last_wcap.end())
+      wcap = it->second;
+  }
+  const bool wait = stream_order && !wcap;
+  u64 total = 0;
+  if (wait) {
+    HIPCHK(hipMemcpyAsync(&total, offs + d.nblocks, sizeof(u64), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    wcap = wcap_for(total);
+    u64 w0 = offset >> 6, w1 = (offset + total + 63) >> 6;
+    if (w1 > w0)
+      hipLaunchKernelGGL(k_prepare, dim3(grid_for(w1 - w0, 256)), dim3(256), 0, st, out, w0, w1,
+                         (u32)(offset & 63));
+  }
+  else {
+    hipLaunchKernelGGL(k_prepare_dev, dim3(std::min(grid, 2048u)), dim3(256), 0, st, out, offset,
+                       (const u64*)(offs + d.nblocks));
+  }
+  if (stream_order) {
     hipLaunchKernelGGL((k_encode_waves<S, D, REV>), dim3(grid), dim3(BS), (size_t)4 * wcap * 8, st, (const S*)data,
                        d.g, d.nblocks, z, out, offset, (const u64*)offs, wcap);
   }
@@ -445,7 +467,14 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
                        d.nblocks, z, out, offset, (const u64*)offs, (u64*)nullptr);
   }
   HIPCHK(hipGetLastError());
+  if (!wait)
+    HIPCHK(hipMemcpyAsync(&total, offs + d.nblocks, sizeof(u64), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (stream_order) {
+    if (last_wcap.size() >= 64)
+      last_wcap.clear();
+    last_wcap[key] = wcap_for(total);
+  }
   *bits_out = total;
   if (boffs_out) {
     *boffs_out = held.p;

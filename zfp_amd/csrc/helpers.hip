@@ -20,6 +20,20 @@ __global__ void k_prepare(u64* out, u64 w0, u64 w1, u32 keep)
     out[i] = 0;
 }
 
+// the same for the words that `*bits` bits from bit `offset` on cover, with
+// the bit count read on the device (grid-stride loop)
+__global__ void k_prepare_dev(u64* out, u64 offset, const u64* bits)
+{
+  const u64 w0 = offset >> 6, w1 = (offset + *bits + 63) >> 6;
+  const u32 keep = (u32)(offset & 63);
+  for (u64 i = w0 + (u64)blockIdx.x * blockDim.x + threadIdx.x; i < w1; i += (u64)gridDim.x * blockDim.x) {
+    if (i == w0 && keep)
+      out[i] &= lowmask(keep);
+    else
+      out[i] = 0;
+  }
+}
+
 
 // ---------------------------------------------------------------------------
 // exclusive prefix sum of u64 block lengths (three passes: per-tile scan,
