@@ -112,16 +112,22 @@ u64 scan_tmp_words(u64 n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
 // one pass over the data (rocPRIM's decoupled look-back scan) with its
 // temporary storage from the grow-only scratch; exclusive_scan_tmp below is
-// the allocation-free variant the batch graphs capture
+// the allocation-free variant the batch graphs capture.  Tiles of 256 x 40
+// values: on 4.2M lengths (a 2D 8191 x 8193 field) 23.5 us on MI355X against
+// 44 us with rocPRIM's default tiles, whose longer look-back chain dominates.
+using ScanTiles = rocprim::scan_config<256, 40, rocprim::block_load_method::block_load_transpose,
+                                       rocprim::block_store_method::block_store_transpose,
+                                       rocprim::block_scan_algorithm::using_warp_scan>;
+
 int exclusive_scan(const u64* in, u64* out, u64 n, hipStream_t st)
 {
   size_t bytes = 0;
-  HIPCHK(rocprim::exclusive_scan(nullptr, bytes, in, out, (u64)0, (size_t)n, rocprim::plus<u64>(), st));
+  HIPCHK(rocprim::exclusive_scan<ScanTiles>(nullptr, bytes, in, out, (u64)0, (size_t)n, rocprim::plus<u64>(), st));
   void* tp;
   int rc = g_scan_tmp.get(bytes ? bytes : 8, &tp);
   if (rc)
     return rc;
-  HIPCHK(rocprim::exclusive_scan(tp, bytes, in, out, (u64)0, (size_t)n, rocprim::plus<u64>(), st));
+  HIPCHK(rocprim::exclusive_scan<ScanTiles>(tp, bytes, in, out, (u64)0, (size_t)n, rocprim::plus<u64>(), st));
   return ZFP_HIP_OK;
 }
 
