@@ -97,14 +97,16 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_sums(u64* sums, u64 ntiles)
   }
 }
 
+// every value of a tile gets the same prefix, so lanes take consecutive
+// words (coalesced) rather than the scan's per-thread runs
 __global__ void __launch_bounds__(SCAN_BS) k_scan_add(u64* out, u64 n, const u64* tile_sums)
 {
-  u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_ITEMS;
+  u64 base = (u64)blockIdx.x * SCAN_TILE + threadIdx.x;
   u64 add = tile_sums[blockIdx.x];
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; i++)
-    if (base + i < n)
-      out[base + i] += add;
+    if (base + (u64)i * SCAN_BS < n)
+      out[base + (u64)i * SCAN_BS] += add;
 }
 
 
