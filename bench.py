@@ -92,16 +92,34 @@ def make_field(torch, n, seed=20251202, special=0.01):
     return f
 
 
-def measured_traffic(dom):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by what the
-    same counters read for a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "k_encode_f64_3d_512_counters.json")
-    if dom != ("compress", 16, "contiguous") or not os.path.exists(path):
+COUNTERS = os.path.join(ROOT, "profiles", "codec_f64_3d_512_counters.json")
+
+
+def measured_traffic(pass_name):
+    """HBM bytes per pass from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE of every kernel of the pass, scaled by what
+    the same counters read for a copy kernel of known size), or None when
+    the summary has no entry for the pass."""
+    if not os.path.exists(COUNTERS):
         return None
-    with open(path) as fh:
+    with open(COUNTERS) as fh:
         c = json.load(fh)
-    return int(c["fetch_bytes_scaled"] + c["write_bytes_scaled"])
+    e = c.get("passes", {}).get(pass_name)
+    if not e:
+        return None
+    return int(e["fetch_bytes"] + e["write_bytes"])
+
+
+def pass_kernels(op, rate, layout):
+    """The kernels one pass launches (codec_kernels.h launch_encode /
+    launch_decode for 3D doubles at a word-aligned fixed rate)."""
+    if op == "compress":
+        if rate <= 16 and layout == "contiguous":
+            return "k_encode<double,3,ENC_FIXED_PLAIN,false,3>"
+        return "k_encode<double,3,ENC_FIXED_PLAIN,false,2>"
+    if rate <= 16:
+        return "k_decode<double,3,STAGE,false,HI_ONLY> + k_decode_list<double,3>"
+    return "k_decode<double,3,STAGE>"
 
 
 def gpu_bench(args):
@@ -196,17 +214,23 @@ def gpu_bench(args):
             "GBps_uncompressed": round(ubytes / (ms * 1e-3) / 1e9, 1),
             "hbm_frac": round((ubytes + cbytes) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         }
-    dom = ("compress", 16, "contiguous")
-    if dom not in per:
-        dom = passes[0]
+    # the dominant pass: the one that takes longest per step
+    dom = max(per, key=lambda p: per[p])
     dom_ms = per[dom]
-    traffic_bytes = ubytes + N * dom[1] // 8
-    achieved = traffic_bytes / (dom_ms * 1e-3) / 1e9
+    dom_name = f"{dom[0]}_rate{dom[1]}_{dom[2]}"
+    alg_bytes = ubytes + N * dom[1] // 8
+    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom_name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
-                "kernel": f"k_encode<double,3,fixed> rate {dom[1]} {dom[2]}" if dom[0] == "compress"
-                          else f"k_decode<double,3> rate {dom[1]} {dom[2]}",
-                "algorithmic_bytes_per_launch": traffic_bytes, "mean_launch_ms": round(dom_ms, 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "pass": dom_name, "kernel": pass_kernels(*dom),
+                "algorithmic_bytes_per_launch": alg_bytes, "mean_launch_ms": round(dom_ms, 4),
+                "traffic_over_algorithmic": round(traffic / alg_bytes, 4) if traffic else None,
+                "min_frac_over_passes": round(min(d["hbm_frac"] for d in detail.values()), 4)}
+    for (op, r, lay), d in zip(per, detail.values()):
+        d["kernels"] = pass_kernels(op, r, lay)
+        t = measured_traffic(f"{op}_rate{r}_{lay}")
+        d["traffic_over_algorithmic"] = round(t / (ubytes + N * r // 8), 4) if t else None
     return value, ms_step, roofline, detail, checks, f
 
 
@@ -414,6 +438,36 @@ def vr_bench(args):
     return res
 
 
+def serve_pool(torch, count, rng):
+    """The serving workload's resident fields: 3D float64 / float32 in turn,
+    edge log-uniform in [32, 384], fixed rate U{4..32} or accuracy
+    10^U(-6,-2), a smooth seeded field plus N(0, 1e-3) noise.  Returns dicts
+    with ztype, n, p (codec parameters), f (device tensor), nw (stream
+    capacity in words), nb (blocks) and var (variable length)."""
+    import zfp_amd as Z
+    pool = []
+    for k in range(count):
+        e = int(round(math.exp(rng.uniform(math.log(32), math.log(384)))))
+        n = [e, e, e]
+        ztype = 4 if k % 2 == 0 else 3
+        if rng.random() < 0.5:
+            p = Z.stream_params("rate", int(rng.integers(4, 33)), ztype, 3)
+        else:
+            p = Z.stream_params("accuracy", float(10.0 ** rng.uniform(-6, -2)), ztype, 3)
+        c = torch.arange(e, device="cuda:0", dtype=torch.float64) * (2 * math.pi / e)
+        f = (torch.sin(3 * c).view(1, 1, e) * torch.cos(2 * c).view(1, e, 1) + torch.sin(c + 1).view(e, 1, 1)
+             + 1e-3 * torch.randn((e, e, e), device="cuda:0", dtype=torch.float64))
+        f = f if ztype == 4 else f.float()
+        pool.append(dict(ztype=ztype, n=n, p=p, f=f, nw=Z.max_stream_words(ztype, n, p),
+                         nb=(e // 4 + (e % 4 > 0)) ** 3, var=p[0] != p[1]))
+    return pool
+
+
+def serve_picks(rng, count, npool):
+    """The request trace: (pool field, "c" compress or "d" decompress), 50/50."""
+    return [(int(rng.integers(0, npool)), "c" if rng.random() < 0.5 else "d") for _ in range(count)]
+
+
 def serve_bench(args):
     """Serving loop: a pool of resident 3D float32/float64 fields (volume
     log-uniform 32^3..384^3; fixed rate U{4..32} or accuracy 10^U(-6,-2)),
@@ -427,21 +481,8 @@ def serve_bench(args):
 
     rng = np.random.default_rng(20251204)
     pool = []
-    for k in range(args.serve_fields):
-        e = int(round(math.exp(rng.uniform(math.log(32), math.log(384)))))
-        n = [e, e, e]
-        ztype = 4 if k % 2 == 0 else 3
-        if rng.random() < 0.5:
-            p = Z.stream_params("rate", int(rng.integers(4, 33)), ztype, 3)
-        else:
-            p = Z.stream_params("accuracy", float(10.0 ** rng.uniform(-6, -2)), ztype, 3)
-        c = torch.arange(e, device="cuda:0", dtype=torch.float64) * (2 * math.pi / e)
-        f = (torch.sin(3 * c).view(1, 1, e) * torch.cos(2 * c).view(1, e, 1) + torch.sin(c + 1).view(e, 1, 1)
-             + 1e-3 * torch.randn((e, e, e), device="cuda:0", dtype=torch.float64))
-        f = f if ztype == 4 else f.float()
-        nw = Z.max_stream_words(ztype, n, p)
-        nb = (e // 4 + (e % 4 > 0)) ** 3
-        var = p[0] != p[1]
+    for P in serve_pool(torch, args.serve_fields, rng):
+        ztype, n, p, f, nw, nb, var = (P[k] for k in ("ztype", "n", "p", "f", "nw", "nb", "var"))
         # reference results: the job alone, one call each
         w = torch.zeros(nw, dtype=torch.int64, device="cuda:0")
         bits = Z.encode(ztype, n, None, f.data_ptr(), p, w.data_ptr())
@@ -461,7 +502,7 @@ def serve_bench(args):
     torch.cuda.synchronize()
 
     R = args.serve_requests
-    picks = [(int(rng.integers(0, len(pool))), "c" if rng.random() < 0.5 else "d") for _ in range(R)]
+    picks = serve_picks(rng, R, len(pool))
     # per-request output buffers, allocated once
     outs = []
     for fi, kind in picks:
@@ -537,6 +578,50 @@ def serve_bench(args):
     }
 
 
+def host_provenance(kind):
+    """What the CPU baseline ran on and how the timed library was built:
+    lscpu's model, sockets, cores and threads, the CPUs this process may
+    use, and the compiler and flags of the reference build (oracle/Makefile's
+    libzfp_ref.so recipe) or of the oracle."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus", "CPU max MHz": "max_mhz"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["usable_cpus"] = os.cpu_count()
+    try:
+        info["compiler"] = subprocess.run(["gcc", "--version"], capture_output=True, text=True,
+                                          timeout=10).stdout.splitlines()[0]
+    except (OSError, subprocess.SubprocessError, IndexError):
+        info["compiler"] = None
+    target = "_ref/libzfp_ref.so:" if kind == "reference" else "liboracle.so:"
+    flags = None
+    try:
+        lines = open(os.path.join(ROOT, "oracle", "Makefile")).read().splitlines()
+        for i, line in enumerate(lines):
+            if line.startswith("$(HERE)" + target):
+                rec = " ".join(x.strip().rstrip("\\") for x in lines[i + 1:i + 4] if x.startswith(("\t", " ")))
+                rec = rec[rec.find("$(CC)"):]
+                if kind != "reference":
+                    rec = rec.replace("$(CFLAGS)", "-O2 -fPIC -std=c99 -ffp-contract=off")
+                flags = " ".join(t for t in rec.split() if t.startswith("-") and not t.startswith(("-I", "-o", "-L")))
+                break
+    except OSError:
+        pass
+    info["flags"] = flags
+    return info
+
+
 def cpu_baseline(args, f_dev):
     """Reference serial zfp on a 256^3 slab of the benchmark field (or the
     oracle when the reference build is absent): one warm-up repetition,
@@ -599,6 +684,7 @@ def cpu_baseline(args, f_dev):
     if oc:
         out["omp_compress_GBps"] = round(nbytes / oc / 1e9, 4)
         out["omp_threads"] = cores
+    out["host"] = host_provenance(kind)
     return out
 
 

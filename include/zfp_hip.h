@@ -75,11 +75,17 @@ int zfp_hip_encode(int type, const size_t n[4], const ptrdiff_t s[4], const void
                    uint64_t* d_words, uint64_t offset, uint64_t* bits_out,
                    void* hip_stream);
 
-/* Device-level decode, the inverse of zfp_hip_encode.  *bits_out receives
-   the number of bits consumed. */
+/* Device-level decode, the inverse of zfp_hip_encode.  d_words holds
+   nwords readable words (the stream's own ceil((offset + bits) / 64) words
+   suffice); no word at or past nwords is read.  *bits_out receives the
+   number of bits consumed.  A variable-rate stream is decoded in parallel
+   with the block index recorded when d_words was last compressed into (or
+   one imported for it), after checking every block's length against that
+   index; when they disagree, or there is no index, the block offsets are
+   recovered from the stream itself.  The result is the same either way. */
 int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4], void* d_data,
                    uint minbits, uint maxbits, uint maxprec, int minexp,
-                   const uint64_t* d_words, uint64_t offset, uint64_t* bits_out,
+                   const uint64_t* d_words, uint64_t nwords, uint64_t offset, uint64_t* bits_out,
                    void* hip_stream);
 
 /*

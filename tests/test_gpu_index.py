@@ -18,8 +18,8 @@ import pytest
 import torch
 
 import zfp_amd as Z
-from zfptest import (ROOT, T_DOUBLE, T_FLOAT, api_compress, have_ref, oracle_compress, oracle_decompress,
-                     ref_api, smooth_field, stream_params_accuracy, stream_params_precision)
+from zfptest import (ROOT, T_DOUBLE, T_FLOAT, api_compress, have_ref, header_prefix, oracle_compress,
+                     oracle_decompress, ref_api, smooth_field, stream_params_accuracy, stream_params_precision)
 
 pytestmark = pytest.mark.gpu
 REF4 = os.path.join(ROOT, "oracle", "_ref", "libhybrid4_ref.so")
@@ -55,7 +55,8 @@ def _decode(ztype, n, p, d_words, out):
     nn = SZ4(*(list(n) + [0] * (4 - len(n))))
     bits = ctypes.c_uint64()
     rc = Z.hip_lib().zfp_hip_decode(ztype, nn, PD4(0, 0, 0, 0), ctypes.c_void_p(out.data_ptr()), *p,
-                                    ctypes.c_void_p(d_words.data_ptr()), 0, ctypes.byref(bits), None)
+                                    ctypes.c_void_p(d_words.data_ptr()), d_words.numel(), 0, ctypes.byref(bits),
+                                    None)
     return rc, bits.value
 
 
@@ -162,4 +163,137 @@ def test_repeated_encodes_into_one_buffer():
         rc, dbits = _decode(ztype, n, p, d_words, out)
         assert rc == 0 and dbits == bits, f"shape {n}"
         assert np.array_equal(out.cpu().numpy().view(np.uint8), ref.view(np.uint8)), f"shape {n}"
+    Z.hip_lib().zfp_hip_index_clear()
+
+
+def _encode(ztype, n, p, d_a, d_words):
+    nn = SZ4(*(list(n) + [0] * (4 - len(n))))
+    ebits = ctypes.c_uint64()
+    assert Z.hip_lib().zfp_hip_encode(ztype, nn, PD4(0, 0, 0, 0), ctypes.c_void_p(d_a.data_ptr()), *p,
+                                      ctypes.c_void_p(d_words.data_ptr()), 0, ctypes.byref(ebits), None) == 0
+    return ebits.value
+
+
+@pytest.mark.parametrize("mode", ["prec16", "acc1e-3"])
+def test_stale_index_after_buffer_overwrite(mode):
+    """Field A is compressed into buffer X (its block index is recorded under
+    X's address); X is then overwritten with serial zfp's stream of a
+    different field B of the same shape and mode.  Decoding X must give
+    serial zfp's decompression of B: the recorded index does not describe
+    X's contents any more, every block's length is checked against it, and
+    the mismatch sends the decode to a parse of the stream."""
+    ztype, n = T_FLOAT, [61, 47]
+    p = stream_params_precision(16) if mode == "prec16" else stream_params_accuracy(1e-3)
+    a = smooth_field(n, ztype, seed=501)
+    b = smooth_field(n, ztype, seed=502, noise=0.3)
+    wb, bits_b = oracle_compress(b, 0, n, None, ztype, p)
+    cap = max(wb.size, oracle_compress(a, 0, n, None, ztype, p)[0].size)
+    d_words = torch.zeros(cap, dtype=torch.int64, device="cuda:0")
+    Z.hip_lib().zfp_hip_index_clear()
+    _encode(ztype, n, p, torch.from_numpy(a).to("cuda:0"), d_words)
+    assert Z.hip_lib().zfp_hip_index_size(ctypes.c_void_p(d_words.data_ptr())) > 0
+    d_words.copy_(torch.from_numpy(np.concatenate([wb, np.zeros(cap - wb.size, np.uint64)]).view(np.int64)))
+    ref = np.zeros_like(b)
+    oracle_decompress(wb, ref, 0, n, None, ztype, p)
+    out = torch.zeros(b.size, dtype=torch.float32, device="cuda:0")
+    rc, dbits = _decode(ztype, n, p, d_words, out)
+    assert rc == 0 and dbits == bits_b
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    # the stale entry is gone; decoding again (now by a parse) agrees
+    assert Z.hip_lib().zfp_hip_index_size(ctypes.c_void_p(d_words.data_ptr())) == 0
+    out.zero_()
+    rc, dbits = _decode(ztype, n, p, d_words, out)
+    assert rc == 0 and dbits == bits_b
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    Z.hip_lib().zfp_hip_index_clear()
+
+
+def test_stale_index_through_zfp_compress():
+    """The same through zfp_compress / zfp_decompress with zfp_exec_hip on a
+    device stream buffer (the index is keyed by the bitstream's begin)."""
+    from hiputil import Run
+    ztype, n, p = T_DOUBLE, [18, 14, 10], stream_params_precision(16)
+    a = smooth_field(n, ztype, seed=511)
+    b = smooth_field(n, ztype, seed=512, noise=0.2)
+    Z.hip_lib().zfp_hip_index_clear()
+    run = Run(a, 0, n, [0, 0, 0], ztype, p)
+    try:
+        assert run.compress() > 0
+        wb, bits_b = oracle_compress(b, 0, n, None, ztype, p)
+        nwb = (bits_b + 63) // 64
+        run.dbuf.zero_()
+        run.dbuf[:nwb].copy_(torch.from_numpy(wb[:nwb].view(np.int64)))
+        ref = np.zeros_like(b)
+        oracle_decompress(wb, ref, 0, n, None, ztype, p)
+        consumed, out = run.decompress(np.zeros_like(b))
+        assert consumed == nwb * 8
+        assert np.array_equal(out.view(np.uint64), ref.view(np.uint64))
+    finally:
+        run.close()
+        Z.hip_lib().zfp_hip_index_clear()
+
+
+def test_zfpy_roundtrips_of_different_streams_in_a_loop():
+    """zfpy.compress_numpy(A) then decompress_numpy(stream of B), repeatedly:
+    NumPy hands freed host buffers out again, so a decompression buffer often
+    sits where an earlier compression wrote a stream of the same shape and
+    mode.  Every result equals serial zfp's decompression of its own stream."""
+    from zfp_amd import zfpy
+    shape = (40, 52)
+    Z.hip_lib().zfp_hip_index_clear()
+    fields = [smooth_field(list(reversed(shape)), T_DOUBLE, seed=530 + k, noise=0.05 * k).reshape(shape)
+              for k in range(4)]
+    streams = [zfpy.compress_numpy(f, precision=16) for f in fields]
+    for it in range(6):
+        for k, f in enumerate(fields):
+            zfpy.compress_numpy(fields[(k + 1 + it) % 4], precision=16)   # registers an index somewhere
+            got = zfpy.decompress_numpy(streams[k])
+            ref = np.zeros(f.size)
+            words = np.frombuffer(streams[k] + b"\0" * (-len(streams[k]) % 8), dtype=np.uint64)
+            _, hbits = header_prefix(T_DOUBLE, list(reversed(shape)), stream_params_precision(16))
+            oracle_decompress(words, ref, 0, list(reversed(shape)), None, T_DOUBLE, stream_params_precision(16),
+                              offset=hbits)
+            assert np.array_equal(got.reshape(-1).view(np.uint64), ref.view(np.uint64)), (it, k)
+    Z.hip_lib().zfp_hip_index_clear()
+
+
+def test_exact_length_buffer():
+    """The decoder reads no word past nwords: a stream held in exactly its own
+    ceil(bits / 64) words decodes like one in a worst-case buffer."""
+    ztype, n, p = T_FLOAT, [33, 29], stream_params_accuracy(1e-4)
+    a = smooth_field(n, ztype, seed=540)
+    wa, bits = oracle_compress(a, 0, n, None, ztype, p)
+    nw = (bits + 63) // 64
+    d_words = torch.from_numpy(wa[:nw].copy().view(np.int64)).to("cuda:0")
+    ref = np.zeros_like(a)
+    oracle_decompress(wa, ref, 0, n, None, ztype, p)
+    out = torch.zeros(a.size, dtype=torch.float32, device="cuda:0")
+    Z.hip_lib().zfp_hip_index_clear()
+    rc, dbits = _decode(ztype, n, p, d_words, out)
+    assert rc == 0 and dbits == bits
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+def test_index_registry_is_bounded():
+    """Indexes recorded under many buffers: the registry keeps the most
+    recently used ones (64 at most) and frees the rest; an evicted stream
+    still decodes (by a parse)."""
+    ztype, n, p = T_FLOAT, [16, 12], stream_params_precision(12)
+    a = smooth_field(n, ztype, seed=550)
+    wa, bits = oracle_compress(a, 0, n, None, ztype, p)
+    d_a = torch.from_numpy(a).to("cuda:0")
+    Z.hip_lib().zfp_hip_index_clear()
+    bufs = [torch.zeros(wa.size, dtype=torch.int64, device="cuda:0") for _ in range(70)]
+    for w in bufs:
+        assert _encode(ztype, n, p, d_a, w) == bits
+    sizes = [Z.hip_lib().zfp_hip_index_size(ctypes.c_void_p(w.data_ptr())) for w in bufs]
+    assert sum(1 for s in sizes if s) == 64
+    assert all(s == 0 for s in sizes[:6]) and all(s > 0 for s in sizes[6:])
+    ref = np.zeros_like(a)
+    oracle_decompress(wa, ref, 0, n, None, ztype, p)
+    for w in (bufs[0], bufs[-1]):
+        out = torch.zeros(a.size, dtype=torch.float32, device="cuda:0")
+        rc, dbits = _decode(ztype, n, p, w, out)
+        assert rc == 0 and dbits == bits
+        assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
     Z.hip_lib().zfp_hip_index_clear()

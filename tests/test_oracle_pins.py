@@ -226,3 +226,37 @@ def test_fixed_rate_size_law(rate):
     p = stream_params_rate(rate, T_DOUBLE, 3)
     _, bits = oracle_compress(a, 0, n, None, T_DOUBLE, p)
     assert bits == blocks_of(n) * p[1]
+
+
+PAR_CASES = [
+    ([37, 29, 23], T_DOUBLE, stream_params_rate(16, T_DOUBLE, 3), None),
+    ([203, 97], T_FLOAT, stream_params_precision(16), None),
+    ([31, 45], T_FLOAT, stream_params_accuracy(1e-3), [45, 1]),          # transposed strides
+    ([9, 10, 11, 13], T_DOUBLE, stream_params_accuracy(1e-7), None),
+    ([301], T_INT64, (1, 16658, 64, ZFP_MIN_EXP - 1), None),             # reversible
+    ([17, 18, 19], T_INT32, (40, 700, 30, ZFP_MIN_EXP), None),           # expert, minbits padding
+]
+
+
+@pytest.mark.parametrize("k", range(len(PAR_CASES)))
+def test_parallel_oracle_equals_single_call(k):
+    """The slab-parallel oracle used on the full-size workloads writes the
+    single-call stream word for word and decodes it to the same values: the
+    blocks of a run of whole block rows are a contiguous part of the stream,
+    so the slab streams concatenated bit for bit are the whole stream."""
+    from zfptest import oracle_compress_parallel, oracle_decompress_parallel, smooth_field
+    n, ztype, p, s = PAR_CASES[k]
+    a = smooth_field(n, ztype, seed=900 + k, special=0.01 if ztype == T_DOUBLE else 0.0)
+    if s:   # the same logical field in a transposed layout
+        a = a.reshape(n[::-1]).T.copy().reshape(-1)
+    words, bits = oracle_compress(a, 0, n, s, ztype, p)
+    for threads in (1, 3, 8):
+        pw, pbits, starts, slabs = oracle_compress_parallel(a, 0, n, s, ztype, p, threads=threads)
+        assert pbits == bits and len(slabs) > (1 if threads > 1 else 0)
+        nw = (bits + 63) // 64
+        assert np.array_equal(pw[:nw], words[:nw])
+        ref = np.zeros_like(a)
+        assert oracle_decompress(words, ref, 0, n, s, ztype, p) == bits
+        out = np.zeros_like(a)
+        assert oracle_decompress_parallel(pw, out, 0, n, s, ztype, p, starts, slabs, threads=threads) == bits
+        assert np.array_equal(out.view(np.uint8), ref.view(np.uint8))

@@ -275,6 +275,92 @@ def oracle_decompress(words, arr_base, elem_offset, n, s, ztype, params, offset=
                                  w.ctypes.data_as(u64p), offset, None)
 
 
+def bit_concat(parts):
+    """Concatenate bit streams: parts is a list of (words, bits) whose words
+    are zero past their bits.  Returns (words, start bit of each part)."""
+    total = sum(int(b) for _, b in parts)
+    out = np.zeros((total + 63) // 64 + 1, dtype=np.uint64)
+    starts = []
+    pos = 0
+    for w, b in parts:
+        b = int(b)
+        starts.append(pos)
+        nw = (b + 63) // 64
+        if nw:
+            w = np.ascontiguousarray(w[:nw], dtype=np.uint64)
+            base, sh = pos // 64, pos % 64
+            if sh == 0:
+                out[base:base + nw] |= w
+            else:
+                out[base:base + nw] |= w << np.uint64(sh)
+                out[base + 1:base + nw + 1] |= w >> np.uint64(64 - sh)
+        pos += b
+    return out, starts
+
+
+def _slabs(n, count):
+    """Ranges [a, b) of the slowest axis that hold whole rows of blocks: the
+    blocks of one range are a contiguous run of the stream (x fastest)."""
+    nb = (n[-1] + 3) // 4
+    k = max(1, min(count, nb))
+    edges = [4 * (nb * i // k) for i in range(k + 1)]
+    edges[-1] = n[-1]
+    return [(edges[i], edges[i + 1]) for i in range(k) if edges[i] < edges[i + 1]]
+
+
+def _strides(n, s):
+    default = [1]
+    for d in range(1, len(n)):
+        default.append(default[-1] * n[d - 1])
+    s = list(s or [])
+    return [s[d] if d < len(s) and s[d] else default[d] for d in range(len(n))]
+
+
+def oracle_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def oracle_compress_parallel(arr_base, elem_offset, n, s, ztype, params, threads=None):
+    """oracle_compress of a large field, in parallel threads over slabs of
+    whole block rows along the slowest axis, each compressed on its own and
+    the slab streams concatenated bit for bit: the stream order runs slab by
+    slab, so the result is the single-call stream.  Returns (words, bits,
+    start bit of each slab, slabs)."""
+    from concurrent.futures import ThreadPoolExecutor
+    threads = threads or oracle_threads()
+    st = _strides(n, s)
+    slabs = _slabs(list(n), 4 * threads)
+
+    def one(r):
+        a, b = r
+        nn = list(n)
+        nn[-1] = b - a
+        return oracle_compress(arr_base, elem_offset + a * st[-1], nn, st, ztype, params)
+
+    with ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(one, slabs))
+    words, starts = bit_concat(parts)
+    return words, sum(int(b) for _, b in parts), starts, slabs
+
+
+def oracle_decompress_parallel(words, arr_base, elem_offset, n, s, ztype, params, starts, slabs, threads=None):
+    """oracle_decompress of the stream oracle_compress_parallel wrote, slab by
+    slab in parallel threads; returns the bits consumed."""
+    from concurrent.futures import ThreadPoolExecutor
+    threads = threads or oracle_threads()
+    st = _strides(n, s)
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+
+    def one(k):
+        a, b = slabs[k]
+        nn = list(n)
+        nn[-1] = b - a
+        return oracle_decompress(w, arr_base, elem_offset + a * st[-1], nn, st, ztype, params, offset=starts[k])
+
+    with ThreadPoolExecutor(threads) as ex:
+        return sum(ex.map(one, range(len(slabs))))
+
+
 def hash64(arr):
     a = np.ascontiguousarray(arr).view(np.uint64)
     return oracle_lib().oracle_hash_words64(a.ctypes.data, a.size, 1)

@@ -153,7 +153,7 @@ _HIP = {
                                       ctypes.c_uint, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
                                       _P(ctypes.c_uint64), ctypes.c_void_p]),
     "zfp_hip_decode": (ctypes.c_int, [ctypes.c_int, _SZ4, _PD4, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
-                                      ctypes.c_uint, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                                      ctypes.c_uint, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                       _P(ctypes.c_uint64), ctypes.c_void_p]),
     "zfp_hip_encode_blocks": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint, ctypes.c_size_t, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_int,
@@ -267,12 +267,16 @@ def encode(ztype, n, s, d_data, params, d_words, offset=0, stream=None):
     return bits.value
 
 
-def decode(ztype, n, s, d_data, params, d_words, offset=0, stream=None):
-    """Device-level decode (zfp_hip_decode); returns bits consumed."""
+def decode(ztype, n, s, d_data, params, d_words, offset=0, stream=None, nwords=None):
+    """Device-level decode (zfp_hip_decode); returns bits consumed.  nwords:
+    readable words at d_words (None: the worst-case stream of the field
+    after `offset`, i.e. a buffer sized like max_stream_words)."""
     nn, ss = _args4(n, s)
+    if nwords is None:
+        nwords = (offset + 63) // 64 + max_stream_words(ztype, n, params)
     bits = ctypes.c_uint64()
     rc = hip_lib().zfp_hip_decode(ztype, nn, ss, ctypes.c_void_p(d_data), *params, ctypes.c_void_p(d_words),
-                                  offset, ctypes.byref(bits), ctypes.c_void_p(stream or 0))
+                                  nwords, offset, ctypes.byref(bits), ctypes.c_void_p(stream or 0))
     if rc:
         raise RuntimeError(f"zfp_hip_decode failed with status {rc}")
     return bits.value

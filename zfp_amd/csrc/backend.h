@@ -86,9 +86,15 @@ struct Scratch {
   }
 };
 
-// scratch of the variable-rate passes (lengths + offsets, scan tiles); all
-// entry points run under one library lock, so one set suffices
-extern Scratch g_lens, g_scan_tmp;
+// scratch of the variable-rate passes (lengths + offsets, scan tiles) and
+// the index-check flag of variable-rate decoding; all entry points run
+// under one library lock, so one set suffices
+extern Scratch g_lens, g_scan_tmp, g_flag;
+
+// internal status of dispatch_decode (never returned by the C ABI): a
+// block's length disagrees with the block index the caller supplied, so the
+// index does not describe this stream (the field may hold partial values)
+constexpr int ZFP_HIP_ESTALE = 100;
 
 // device buffer for a stream's block index of at least `bytes`: the spare
 // buffer of the index registry when it is large enough, else a new

@@ -225,11 +225,17 @@ k_encode_waves(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* _
 // blocks that go on below plane 32 to redo[] (*nredo counts them, one atomic
 // per wave); k_decode_list then decodes those in full, over the partial
 // values written here.
+//
+// bad (variable length with a block index): set to 1 when a block consumes
+// other than boffs[b+1] - boffs[b] bits.  Block 0 starts at the stream's
+// start, so when no block disagrees every offset is, by induction, the true
+// start of its block; one disagreement means the index does not belong to
+// this stream and the caller decodes again from a parse.
 template <typename S, int D, bool STAGE, bool REV = false, bool HI_ONLY = false>
 __global__ void __launch_bounds__(256, HI_ONLY ? 3 : 2)
 k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restrict__ in,
          u64 nwords, u64 offset, const u64* __restrict__ boffs, u32 stride,
-         u32* __restrict__ redo = nullptr, u32* __restrict__ nredo = nullptr)
+         u32* __restrict__ redo = nullptr, u32* __restrict__ nredo = nullptr, u32* __restrict__ bad = nullptr)
 {
   extern __shared__ u64 stage[];
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -280,7 +286,9 @@ k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restr
     }
   }
   else {
-    decode_any<S, D, true, REV>(r, data, g, bc, z);
+    const u32 c = decode_any<S, D, true, REV>(r, data, g, bc, z);
+    if (bad && (u64)c != boffs[b + 1] - boffs[b])
+      *bad = 1u;
   }
 }
 
@@ -492,6 +500,17 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
   const bool rev = is_reversible(z);
   const bool fixed = !rev && z.minbits == z.maxbits;
   u64 total = 0;
+  // a variable-length stream decoded with the caller's index checks every
+  // block against it (k_decode's `bad`)
+  u32* bad = nullptr;
+  if (!fixed && boffs) {
+    void* fp;
+    int rc = g_flag.get(sizeof(u64), &fp);
+    if (rc)
+      return rc;
+    bad = (u32*)fp;
+    HIPCHK(hipMemsetAsync(bad, 0, sizeof(u32), st));
+  }
   if (!fixed && !boffs) {
     // no index: recover block offsets by a sequential parse
     void* lp;
@@ -550,18 +569,23 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
   }
   else if (rev) {
     hipLaunchKernelGGL((k_decode<S, D, false, true>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z,
-                       in, nwords, offset, boffs, 0u);
+                       in, nwords, offset, boffs, 0u, (u32*)nullptr, (u32*)nullptr, bad);
   }
   else {
     hipLaunchKernelGGL((k_decode<S, D, false>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z, in,
-                       nwords, offset, boffs, 0u);
+                       nwords, offset, boffs, 0u, (u32*)nullptr, (u32*)nullptr, bad);
   }
   HIPCHK(hipGetLastError());
+  u32 mismatch = 0;
   if (fixed)
     total = (u64)d.nblocks * z.maxbits;
   else
     HIPCHK(hipMemcpyAsync(&total, boffs + d.nblocks, sizeof(u64), hipMemcpyDeviceToHost, st));
+  if (bad)
+    HIPCHK(hipMemcpyAsync(&mismatch, bad, sizeof(u32), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (mismatch)
+    return ZFP_HIP_ESTALE;
   *bits_out = total;
   return ZFP_HIP_OK;
 }

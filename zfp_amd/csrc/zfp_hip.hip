@@ -16,7 +16,7 @@
 
 namespace zfphip {
 
-Scratch g_lens, g_scan_tmp;
+Scratch g_lens, g_scan_tmp, g_flag;
 
 namespace {
 thread_local char g_last_error[256] = "";
@@ -123,8 +123,11 @@ namespace {
 
 // Block index of a variable-rate stream: offsets[b] relative to `start`
 // (nblocks + 1 entries, on the device).  Keyed by the stream buffer's begin
-// pointer; an entry matches a decode only at the same start offset, block
-// count and codec parameters (type/dims/extents too unless imported).
+// pointer; an entry is tried for a decode only at the same start offset,
+// block count and codec parameters (type/dims/extents too unless imported).
+// The key says nothing about the buffer's contents -- another stream may
+// since have been written there -- so every decode checks each block's
+// length against the entry and drops the entry on a mismatch.
 struct Index {
   u64 start = 0;
   u32 nblocks = 0;
@@ -133,8 +136,15 @@ struct Index {
   Params z{};
   bool any_params = false;
   u64* d_offsets = nullptr;
+  u64 stamp = 0;          // last use, for eviction
 };
 std::map<const void*, Index> g_index;
+u64 g_index_clock = 0;
+
+// Bounds of the registry: entries beyond them are evicted, least recently
+// used first (a later decode of an evicted stream parses it instead).
+constexpr size_t INDEX_MAX_ENTRIES = 64;
+constexpr u64 INDEX_MAX_BYTES = 1ull << 31;
 
 // one index buffer that left the registry, kept for the next variable-rate
 // encode: compressing again and again into one stream buffer then allocates
@@ -142,12 +152,36 @@ std::map<const void*, Index> g_index;
 u64* g_spare = nullptr;
 u64 g_spare_bytes = 0;
 
-void index_put(const void* key, const Index& ix)
+u64 index_bytes(const Index& ix) { return ((u64)ix.nblocks + 1) * sizeof(u64); }
+
+void index_drop(const void* key)
 {
   auto it = g_index.find(key);
-  if (it != g_index.end() && it->second.d_offsets)
-    index_release(it->second.d_offsets, ((u64)it->second.nblocks + 1) * sizeof(u64));
-  g_index[key] = ix;
+  if (it == g_index.end())
+    return;
+  if (it->second.d_offsets)
+    index_release(it->second.d_offsets, index_bytes(it->second));
+  g_index.erase(it);
+}
+
+void index_put(const void* key, const Index& ix)
+{
+  index_drop(key);
+  Index e = ix;
+  e.stamp = ++g_index_clock;
+  g_index[key] = e;
+  for (;;) {
+    u64 bytes = 0;
+    auto old = g_index.end();
+    for (auto it = g_index.begin(); it != g_index.end(); ++it) {
+      bytes += index_bytes(it->second);
+      if (it->first != key && (old == g_index.end() || it->second.stamp < old->second.stamp))
+        old = it;
+    }
+    if ((g_index.size() <= INDEX_MAX_ENTRIES && bytes <= INDEX_MAX_BYTES) || old == g_index.end())
+      break;
+    index_drop(old->first);
+  }
 }
 
 const Index* index_find(const void* key, u64 start, const Desc& d, const size_t* n, const Params& z)
@@ -155,7 +189,7 @@ const Index* index_find(const void* key, u64 start, const Desc& d, const size_t*
   auto it = g_index.find(key);
   if (it == g_index.end())
     return nullptr;
-  const Index& ix = it->second;
+  Index& ix = it->second;
   if (ix.start != start || ix.nblocks != d.nblocks)
     return nullptr;
   if (ix.type) {
@@ -168,7 +202,22 @@ const Index* index_find(const void* key, u64 start, const Desc& d, const size_t*
   if (!ix.any_params && (ix.z.minbits != z.minbits || ix.z.maxbits != z.maxbits ||
                          ix.z.maxprec != z.maxprec || ix.z.minexp != z.minexp))
     return nullptr;
+  ix.stamp = ++g_index_clock;
   return &ix;
+}
+
+// decode with the registry's index for `key` when one applies; an index
+// that disagrees with the stream is dropped and the stream parsed instead
+int decode_indexed(const void* key, const Desc& d, void* data, const size_t* n, const Params& z, const u64* in,
+                   u64 nwords, u64 offset, u64 start, u64* bits, hipStream_t st)
+{
+  const Index* ix = fixed_rate(z) || !index_matches_decoder(z) ? nullptr : index_find(key, start, d, n, z);
+  int rc = dispatch_decode(d, data, z, in, nwords, offset, ix ? ix->d_offsets : nullptr, bits, st);
+  if (rc == ZFP_HIP_ESTALE) {
+    index_drop(key);
+    rc = dispatch_decode(d, data, z, in, nwords, offset, nullptr, bits, st);
+  }
+  return rc;
 }
 
 int g_dev_state = -1; // -1 unknown, 0 usable, else an error code
@@ -377,7 +426,7 @@ extern "C" int zfp_hip_encode(int type, const size_t n[4], const ptrdiff_t s[4],
 
 extern "C" int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4], void* d_data,
                               uint minbits, uint maxbits, uint maxprec, int minexp,
-                              const uint64_t* d_words, uint64_t offset, uint64_t* bits_out,
+                              const uint64_t* d_words, uint64_t nwords, uint64_t offset, uint64_t* bits_out,
                               void* hip_stream)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
@@ -390,18 +439,15 @@ extern "C" int zfp_hip_decode(int type, const size_t n[4], const ptrdiff_t s[4],
     return rc;
   if (minbits > maxbits || !maxprec || maxprec > 64 || !d_data || !d_words)
     return record_error(ZFP_HIP_EINVAL, "invalid codec parameters or null pointer");
+  if (nwords <= (offset >> 6))
+    return record_error(ZFP_HIP_EINVAL, "the stream ends before its first block");
   Params z{minbits, maxbits, maxprec, minexp};
-  const u64* boffs = nullptr;
-  if (!fixed_rate(z) && index_matches_decoder(z)) {
-    const Index* ix = index_find(d_words, offset, d, n, z);
-    if (ix)
-      boffs = ix->d_offsets;
-  }
-  // the caller's buffer holds at least the worst-case stream; reads beyond
-  // it are never issued
-  u64 nwords = (offset + (u64)d.nblocks * max_block_bits(d, z) + 63) >> 6;
+  // reads stop at the caller's nwords words, or at the end of the
+  // worst-case stream when that comes first; a stream cut short decodes its
+  // missing bits as zeros
+  nwords = std::min<u64>(nwords, (offset + (u64)d.nblocks * max_block_bits(d, z) + 63) >> 6);
   u64 bits = 0;
-  rc = dispatch_decode(d, d_data, z, d_words, nwords, offset, boffs, &bits, (hipStream_t)hip_stream);
+  rc = decode_indexed(d_words, d, d_data, n, z, d_words, nwords, offset, offset, &bits, (hipStream_t)hip_stream);
   if (rc)
     return rc;
   if (bits_out)
@@ -519,8 +565,8 @@ extern "C" int zfp_hip_decode_blocks(int type, unsigned dims, size_t count, void
   if (rc)
     return rc;
   if (!d_slots)
-    return zfp_hip_decode(type, n, s, d_blocks, minbits, maxbits, maxprec, minexp, d_words, offset, bits_out,
-                          hip_stream);
+    return zfp_hip_decode(type, n, s, d_blocks, minbits, maxbits, maxprec, minexp, d_words, nwords, offset,
+                          bits_out, hip_stream);
   std::lock_guard<std::mutex> lock(g_mutex);
   if ((rc = device_check()))
     return rc;
@@ -651,12 +697,8 @@ extern "C" size_t zfp_hip_decompress(zfp_stream* zs, zfp_field* field)
   const u64 word0 = rpos >> 6;
   const u32 lead = (u32)(rpos & 63);
   Params z{zs->minbits, zs->maxbits, zs->maxprec, zs->minexp};
-  const bool fixed = fixed_rate(z);
   hipStream_t st = nullptr;
   hipError_t e;
-
-  const Index* ix = fixed || !index_matches_decoder(z) ? nullptr : index_find(bs->begin, rpos, d, n, z);
-  const u64* boffs = ix ? ix->d_offsets : nullptr;
 
   // readable words: up to the end of the stream buffer, or of the stream
   u64 avail = bs->end > bs->begin + word0 ? (u64)(bs->end - bs->begin) - word0 : 0;
@@ -694,7 +736,7 @@ extern "C" size_t zfp_hip_decompress(zfp_stream* zs, zfp_field* field)
     d_data = (char*)dbuf - lo * (i64)d.esize;
   }
   u64 bits = 0;
-  if (dispatch_decode(d, d_data, z, d_in, need, lead, boffs, &bits, st))
+  if (decode_indexed(bs->begin, d, d_data, n, z, d_in, need, lead, rpos, &bits, st))
     return 0;
   if (hbegin && (e = hipMemcpy(hbegin, dbuf, span_bytes, hipMemcpyDeviceToHost)) != hipSuccess)
     return hip_failed(e, "copy of the field to the host");
