@@ -116,6 +116,31 @@ uint64_t dec_split(void* data, const size_t* n, const ptrdiff_t* s, const Params
   return (uint64_t)nb * z.maxbits;
 }
 
+// fixed rate with word-aligned blocks through the slot writer and its
+// straight-line plane coder: each block coded into its own zeroed slot of
+// maxbits/64 + 2 words, whose first maxbits/64 words are the block's
+template <typename S, int D>
+uint64_t enc_slot(const void* data, const size_t* n, const ptrdiff_t* s, const Params& z, uint64_t* words,
+                  uint64_t offset)
+{
+  Geom g;
+  u32 nb;
+  make_geom(D, n, s, g, nb);
+  const u32 nw = z.maxbits / 64;
+  std::vector<uint64_t> slot(nw + 2);
+  for (u32 b = 0; b < nb; b++) {
+    u32 bc[4] = {0, 0, 0, 0};
+    block_coords<D>(b, g, bc);
+    std::fill(slot.begin(), slot.end(), 0ull);
+    SlotWriter w;
+    w.begin(slot.data(), nw + 2);
+    encode_block<S, D>(w, (const S*)data, g, bc, z);
+    for (u32 i = 0; i < nw; i++)
+      words[offset / 64 + (uint64_t)b * nw + i] = slot[i];
+  }
+  return (uint64_t)nb * z.maxbits;
+}
+
 template <typename S>
 uint64_t enc_t(int dims, const void* data, const size_t* n, const ptrdiff_t* s, const Params& z,
                uint64_t* words, uint64_t offset)
@@ -168,6 +193,28 @@ extern "C" uint64_t hostcodec_decompress(int type, const size_t* n, const ptrdif
     case 3: return dec_t<float>(dims, data, n, s, z, words, nwords, offset);
     default: return dec_t<double>(dims, data, n, s, z, words, nwords, offset);
   }
+}
+
+// Fixed-rate 1-3D fields with maxbits and offset multiples of 64 through the
+// slot writer (the LDS-slot encoder); returns 0 for other requests.
+extern "C" uint64_t hostcodec_compress_slot(int type, const size_t* n, const ptrdiff_t* s, const void* data,
+                                            unsigned maxbits, unsigned maxprec, int minexp, uint64_t* words,
+                                            uint64_t offset)
+{
+  int dims = n[0] ? n[1] ? n[2] ? n[3] ? 4 : 3 : 2 : 1 : 0;
+  Params z{maxbits, maxbits, maxprec, minexp};
+  if (dims < 1 || dims > 3 || (maxbits & 63) || (offset & 63))
+    return 0;
+#define ZH_SLOT(S) \
+  return dims == 1 ? enc_slot<S, 1>(data, n, s, z, words, offset) \
+       : dims == 2 ? enc_slot<S, 2>(data, n, s, z, words, offset) : enc_slot<S, 3>(data, n, s, z, words, offset)
+  switch (type) {
+    case 1: ZH_SLOT(i32);
+    case 2: ZH_SLOT(i64);
+    case 3: ZH_SLOT(float);
+    default: ZH_SLOT(double);
+  }
+#undef ZH_SLOT
 }
 
 // Fixed-rate int64/double fields of 1-3 dimensions through the two-pass

@@ -141,3 +141,41 @@ def test_split_decode_matches_full(shape, tname, rate):
         if rate << (2 * len(n)) >= 128:
             assert nredo.value > 0
         assert np.array_equal(out.view(np.uint8), ref.view(np.uint8))
+
+
+SLOT_RATES = [1, 2, 4, 8, 16, 32, 48]
+
+
+@pytest.mark.parametrize("rate", SLOT_RATES)
+@pytest.mark.parametrize("tname", list(TYPES))
+@pytest.mark.parametrize("shape", ["1d", "2d", "3d", "3d_full"])
+def test_slot_encoder_matches_oracle(shape, tname, rate):
+    """The LDS-slot fixed-rate encoder (straight-line plane coder, write-
+    through slot writer) writes serial zfp's stream, including blocks with
+    special values and the permuted negative-stride layout."""
+    n = SHAPES[shape]
+    t = TYPES[tname]
+    dims = len(n)
+    bits_per_block = rate << (2 * dims)
+    if bits_per_block % 64 or (t in (T_FLOAT, T_DOUBLE) and bits_per_block < (9 if t == T_FLOAT else 12)):
+        pytest.skip("not a word-aligned rate")
+    p = stream_params_rate(rate, t, dims)
+    assert p[1] % 64 == 0
+    a = smooth_field(n, t, seed=zlib.crc32(f"slot/{shape}/{tname}".encode()) & 0xffff,
+                     special=0.02 if t in (T_FLOAT, T_DOUBLE) else 0.0)
+    f = lib().hostcodec_compress_slot
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_int, szp, pdp, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_int, u64p,
+                  ctypes.c_uint64]
+    for layout, offset in (("as_is", 0), ("permuted_neg", 128)):
+        if layout != "as_is" and dims == 1:
+            continue
+        base, off, s = make_layout(a, n, layout)
+        ew, eb = oracle_compress(base, off, n, s, t, p, offset=offset)
+        hw = np.zeros_like(ew)
+        hw[: offset // 64] = ew[: offset // 64]
+        nn, ss = field_args(n, s)
+        hb = f(t, nn, ss, ctypes.c_void_p(base.ctypes.data + off * base.itemsize), p[1], p[2], p[3],
+               hw.ctypes.data_as(u64p), offset)
+        assert hb == eb
+        assert np.array_equal(hw, ew)

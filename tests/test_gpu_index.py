@@ -275,25 +275,37 @@ def test_exact_length_buffer():
 
 
 def test_index_registry_is_bounded():
-    """Indexes recorded under many buffers: the registry keeps the most
-    recently used ones (64 at most) and frees the rest; an evicted stream
-    still decodes (by a parse)."""
+    """Indexes recorded under many host stream buffers (zfp_compress into host
+    memory): the registry keeps the 64 most recently used and frees the rest;
+    an evicted stream still decodes (by a parse).  Device buffers are kept in
+    far larger numbers (a batch of 1024 arrays compressed one call each keeps
+    all 1024)."""
+    from hiputil import Run
     ztype, n, p = T_FLOAT, [16, 12], stream_params_precision(12)
     a = smooth_field(n, ztype, seed=550)
     wa, bits = oracle_compress(a, 0, n, None, ztype, p)
-    d_a = torch.from_numpy(a).to("cuda:0")
     Z.hip_lib().zfp_hip_index_clear()
-    bufs = [torch.zeros(wa.size, dtype=torch.int64, device="cuda:0") for _ in range(70)]
+    runs = [Run(a, 0, n, [0, 0], ztype, p, device_stream=False) for _ in range(70)]
+    try:
+        for r in runs:
+            assert r.compress() == (bits + 63) // 64 * 8
+        keys = [ctypes.c_void_p(r.hbuf.ctypes.data) for r in runs]
+        sizes = [Z.hip_lib().zfp_hip_index_size(k) for k in keys]
+        assert sum(1 for s in sizes if s) == 64
+        assert all(s == 0 for s in sizes[:6]) and all(s > 0 for s in sizes[6:])
+        ref = np.zeros_like(a)
+        oracle_decompress(wa, ref, 0, n, None, ztype, p)
+        for r in (runs[0], runs[-1]):
+            consumed, out = r.decompress(np.zeros_like(a))
+            assert consumed == (bits + 63) // 64 * 8
+            assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+    finally:
+        for r in runs:
+            r.close()
+    # device buffers: 200 kept
+    d_a = torch.from_numpy(a).to("cuda:0")
+    bufs = [torch.zeros(wa.size, dtype=torch.int64, device="cuda:0") for _ in range(200)]
     for w in bufs:
         assert _encode(ztype, n, p, d_a, w) == bits
-    sizes = [Z.hip_lib().zfp_hip_index_size(ctypes.c_void_p(w.data_ptr())) for w in bufs]
-    assert sum(1 for s in sizes if s) == 64
-    assert all(s == 0 for s in sizes[:6]) and all(s > 0 for s in sizes[6:])
-    ref = np.zeros_like(a)
-    oracle_decompress(wa, ref, 0, n, None, ztype, p)
-    for w in (bufs[0], bufs[-1]):
-        out = torch.zeros(a.size, dtype=torch.float32, device="cuda:0")
-        rc, dbits = _decode(ztype, n, p, w, out)
-        assert rc == 0 and dbits == bits
-        assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    assert all(Z.hip_lib().zfp_hip_index_size(ctypes.c_void_p(w.data_ptr())) for w in bufs)
     Z.hip_lib().zfp_hip_index_clear()

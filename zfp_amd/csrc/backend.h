@@ -107,21 +107,32 @@ void index_release(u64* p, u64 bytes);
 // for D = 1..4 in inst_d<D>.hip.  On variable-rate encodes, *boffs (when
 // non-null) receives the nblocks+1 block offsets in a buffer from
 // index_buffer, which the caller owns.
+// Device scratch of a variable-rate encode that the caller provides instead
+// of the library's grow-only buffers: block lengths and offsets of nblocks + 1
+// words each, and scan_tmp_words(nblocks + 1) words of scan tile sums.  With
+// it the encode allocates nothing and records no block index.
+struct Workspace {
+  u64* lens;
+  u64* offs;
+  u64* tmp;
+};
+
 template <int D>
 int dispatch_encode_d(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
-                      u64* bits, hipStream_t st, u64** boffs, const u64* at = nullptr);
+                      u64* bits, hipStream_t st, u64** boffs, const u64* at = nullptr,
+                      const Workspace* ws = nullptr);
 template <int D>
 int dispatch_decode_d(const Desc& d, void* data, const Params& z, const u64* in, u64 nwords,
                       u64 offset, const u64* boffs, u64* bits, hipStream_t st);
 
 extern template int dispatch_encode_d<1>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
-                                           const u64*);
+                                           const u64*, const Workspace*);
 extern template int dispatch_encode_d<2>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
-                                           const u64*);
+                                           const u64*, const Workspace*);
 extern template int dispatch_encode_d<3>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
-                                           const u64*);
+                                           const u64*, const Workspace*);
 extern template int dispatch_encode_d<4>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
-                                           const u64*);
+                                           const u64*, const Workspace*);
 extern template int dispatch_decode_d<1>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
 extern template int dispatch_decode_d<2>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
 extern template int dispatch_decode_d<3>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);

@@ -45,9 +45,8 @@ __device__ __forceinline__ void block_coords(u32 b, const Geom& g, u32* c)
 // around the lanes' own (divergent) group-test loops makes the compiler
 // treat the plane number as varying across lanes, and index the plane
 // vectors lane by lane.  A plane no lane wants costs a vote and a branch.
-template <int N, int P, typename UInt, typename W>
-__device__ __forceinline__ void encode_half(PlaneEncoder<N>& pe, W& w, const PlaneVec& lo, const PlaneVec& hi,
-                                            u32 base)
+template <int N, int P, typename UInt, typename PE, typename W>
+__device__ __forceinline__ void encode_half(PE& pe, W& w, const PlaneVec& lo, const PlaneVec& hi, u32 base)
 {
   for (int k = 31; k >= 0; k--) {
     const bool act = pe.wants(base + (u32)k);
@@ -63,7 +62,8 @@ __device__ __forceinline__ void encode_half(PlaneEncoder<N>& pe, W& w, const Pla
 template <int N, int P, typename UInt, typename W>
 __device__ __forceinline__ u32 encode_planes(W& w, const UInt* u, u32 maxbits, u32 maxprec)
 {
-  PlaneEncoder<N> pe;
+  // a slot writer takes the straight-line coder
+  typename std::conditional<std::is_same<W, SlotWriter>::value, SlotPlaneEncoder<N>, PlaneEncoder<N>>::type pe;
   pe.init(P, maxbits, maxprec);
   PlaneVec lo, hi;
   if constexpr (P == 64) {

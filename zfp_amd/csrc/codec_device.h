@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace zfphip {
 
 typedef uint64_t u64;
@@ -661,6 +663,46 @@ struct BitWriter {
   }
 };
 
+// Bit output into a lane's own run of words (a slot of LDS that is zero
+// beforehand).  Every put stores both words it can touch, so the slot always
+// holds every bit put so far: nothing is pending and no put branches on
+// whether a word filled up.  The slot has two words past the block's own;
+// puts beyond them (a budgeted coder's last plane) stay in those two.
+struct SlotWriter {
+  u64* slot;
+  u64 acc;    // slot[wi]
+  u32 wi;     // word being filled
+  u32 fill;   // bits in it (< 64)
+  u32 last;   // highest value of wi (slot words - 2)
+
+  ZH_INLINE void begin(u64* s, u32 words)
+  {
+    slot = s; acc = 0; wi = 0; fill = 0; last = words - 2;
+  }
+  // append the low n (<= 64) bits of v; v must have no bits at or above n
+  ZH_INLINE void put(u64 v, u32 n)
+  {
+    const u64 lo = acc | (v << fill);
+    const u64 hi = (v >> 1) >> (63 - fill); // bits of v past the word (none when fill = 0)
+    slot[wi] = lo;
+    slot[wi + 1] = hi;
+    const u32 nf = fill + n;
+    const bool full = nf >= 64;
+    acc = full ? hi : lo;
+    wi = umin(wi + (full ? 1u : 0u), last);
+    fill = nf & 63u;
+  }
+  // append n zero bits (the slot's words past wi are zero already)
+  ZH_INLINE void zeros(u64 n)
+  {
+    const u64 nf = fill + n;
+    const u64 adv = nf >> 6;
+    acc = adv ? 0ull : acc;
+    wi = (u32)(adv >= (u64)(last - wi) ? last : wi + adv);
+    fill = (u32)(nf & 63);
+  }
+};
+
 // Counts bits without writing (length pass of variable-rate modes).
 struct BitCounter {
   u64 pos;
@@ -817,6 +859,79 @@ struct PlaneEncoder {
         w.put((c1 >> 1) >> (63 - n0), L - 128);
     }
     pos += L;
+  }
+};
+
+// The same code written through a SlotWriter, with the same instructions for
+// every case of a plane -- all coefficients significant, no new one-bit, new
+// one-bits -- so that the lanes of a wave never split on them.  The n
+// verbatim bits go out as one put, the group-test code as one (two when the
+// code of some lane of the wave is longer than 64 bits).  The code's one-bits
+// are placed by a loop over the plane's new one-bits, whose trip count is the
+// largest in the wave.
+template <int N>
+struct SlotPlaneEncoder {
+  u32 n;      // coefficients already significant
+  u32 pos;    // bits of unbudgeted code written
+  u32 limit;  // budget (0xffffffff when it cannot bind)
+  u32 kmin;   // lowest plane coded
+
+  ZH_INLINE void init(u32 P, u32 maxbits, u32 maxprec)
+  {
+    n = 0;
+    pos = 0;
+    kmin = P > maxprec ? P - maxprec : 0;
+    limit = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
+  }
+  ZH_INLINE bool wants(u32 k) const { return k >= kmin && pos < limit; }
+  ZH_INLINE u32 length() const { return umin(pos, limit); }
+
+  ZH_INLINE void plane(SlotWriter& w, u64 x)
+  {
+    // the n significant coefficients' bits verbatim; r: the others' bits
+    const u64 sig = lowmask(n);
+    const u64 r = (x & ~sig) >> (n & 63u);
+    const bool all = n >= (u32)N;
+    const bool nz = r != 0;
+    const u32 m = (u32)__builtin_popcountll(r);
+    const u32 pm = nz ? 63u - (u32)__builtin_clzll(r) : 0u;
+    const bool implied = nz && n + pm + 1 >= (u32)N;
+    // group-test code: bit 0 the first test; the j-th one-bit of r (at p)
+    // at p + j, then the next test (a 1) at p + j + 1; the last one-bit is
+    // followed by a 0 test, or is itself implied (not written) when it is
+    // the last coefficient
+    const u32 keep = nz ? (implied ? pm + m : pm + m + 1) : 0u;
+    const u32 glen = all ? 0u : !nz ? 1u : (implied ? pm + m : pm + m + 2);
+    u64 c0 = nz ? 1ull : 0ull, c1 = 0;
+    if (wave_any(nz && pm + m >= 63)) {
+      // some lane's code reaches past bit 63
+      u64 rr = r;
+      for (u32 j = 1; wave_any(rr != 0); j++) {
+        if (rr) {
+          const u32 s = ctz64(rr) + j;   // < 128
+          xor128(c0, c1, 3ull, s);
+          rr &= rr - 1;
+        }
+      }
+      c1 &= lowmask(keep > 64 ? keep - 64 : 0u);
+      c0 &= lowmask(keep);
+    }
+    else {
+      u64 rr = r;
+      for (u32 j = 1; wave_any(rr != 0); j++) {
+        if (rr) {
+          c0 |= 3ull << (ctz64(rr) + j);   // < 64
+          rr &= rr - 1;
+        }
+      }
+      c0 &= lowmask(keep);
+    }
+    w.put(x & sig, all ? (u32)N : n);
+    w.put(c0, umin(glen, 64u));
+    if (wave_any(glen > 64))
+      w.put(c1, glen > 64 ? glen - 64 : 0u);
+    pos += (all ? (u32)N : n) + glen;
+    n += nz ? pm + 1 : 0u;
   }
 };
 

@@ -74,6 +74,8 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
     BitCounter cnt;
     cnt.pos = 0;
     lens[b] = encode_any<S, D, REV>(cnt, data, g, bc, z);
+    if (t == 0)
+      lens[nblocks] = 0; // so that the exclusive scan's last entry is the total
     return;
   }
   u64 start, end;
@@ -93,6 +95,72 @@ k_encode(const S* __restrict__ data, Geom g, u32 nblocks, Params z,
   w.begin(out, start, end);
   encode_any<S, D, REV>(w, data, g, bc, z);
   w.finish();
+}
+
+// LDS operations of the lanes of one wave, ordered before what follows
+ZH_INLINE void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Fixed rate with word-aligned blocks (maxbits = 64 * nw, offset a multiple
+// of 64), 1-3D: each lane codes its block into its own slot of LDS (nw words
+// plus two for the last plane's overflow, slot stride `stride`, odd), with
+// the straight-line plane coder; then the block's nw words go to the stream.
+// When the lanes hold consecutive stream blocks (lane order = stream order)
+// the wave's blocks are one run of words, copied with coalesced stores;
+// otherwise each lane stores its own block's words.
+template <typename S, int D, int OCC>
+__global__ void __launch_bounds__(256, OCC)
+k_encode_slot(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* __restrict__ out, u64 word0,
+              u32 stride, u32 stream_order)
+{
+  extern __shared__ u64 slots[];
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 lane = threadIdx.x & 63u;
+  const u32 t0 = t - lane;
+  if (t0 >= nblocks)
+    return; // the whole wave lies past the field
+  const u32 nw = z.maxbits >> 6;
+  u64* slot = slots + (u64)threadIdx.x * stride;
+  for (u32 i = 0; i < nw + 2; i++)
+    slot[i] = 0;
+  u32 b = 0;
+  if (t < nblocks) {
+    u32 bc[4] = {0, 0, 0, 0};
+    b = lane_block<D>(t, g, bc);
+    SlotWriter w;
+    w.begin(slot, nw + 2);
+    encode_block<S, D>(w, data, g, bc, z);
+  }
+  if (stream_order) {
+    // blocks t0 .. t0 + cnt - 1 are stream blocks t0 ..: words
+    // word0 + t0 * nw .. of the stream; word q of the run is word q % nw of
+    // the slot of lane q / nw
+    wave_lds_sync();
+    const u32 cnt = umin(64u, nblocks - t0);
+    const u32 total = cnt * nw;
+    const u64* wslots = slots + (u64)(threadIdx.x - lane) * stride;
+    u64* dst = out + word0 + (u64)t0 * nw;
+    u32 blk = lane / nw, wd = lane - blk * nw;
+    const u32 db = 64u / nw, dw = 64u - db * nw;
+    for (u32 q = lane; q < total; q += 64) {
+      dst[q] = wslots[blk * stride + wd];
+      wd += dw;
+      blk += db;
+      if (wd >= nw) {
+        wd -= nw;
+        blk++;
+      }
+    }
+  }
+  else if (t < nblocks) {
+    u64* dst = out + word0 + (u64)b * nw;
+    for (u32 i = 0; i < nw; i++)
+      dst[i] = slot[i];
+  }
 }
 
 // Bit writer of the variable-length write pass whose wave assembles its
@@ -158,14 +226,6 @@ struct WaveWriter {
       emit(acc);
   }
 };
-
-// LDS operations of the lanes of one wave, ordered before what follows
-ZH_INLINE void wave_lds_sync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Write pass of variable-length coding when lane t holds stream block t (the
 // lane order is the stream's): each wave zeroes its LDS area (wcap words),
@@ -336,21 +396,21 @@ static inline bool is_reversible(const Params& z) { return z.minexp < -1074; }
 
 template <typename S, int D, bool REV>
 int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits_out,
-                      hipStream_t st, u64** boffs_out);
+                      hipStream_t st, u64** boffs_out, const Workspace* ws);
 
 // at (fixed rate, word-aligned blocks only): block b is written at bit
 // offset + at[b] instead of offset + b * maxbits, and no other word is touched
 template <typename S, int D>
 int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
-                         u64* bits_out, hipStream_t st, u64** boffs_out, const u64* at)
+                         u64* bits_out, hipStream_t st, u64** boffs_out, const u64* at, const Workspace* ws)
 {
   const unsigned BS = 256;
   const unsigned grid = grid_for(d.nblocks, BS);
   if (is_reversible(z))
-    return launch_encode_var<S, D, true>(d, data, z, out, offset, bits_out, st, boffs_out);
+    return launch_encode_var<S, D, true>(d, data, z, out, offset, bits_out, st, boffs_out, ws);
   const bool fixed = z.minbits == z.maxbits;
   if (!fixed)
-    return launch_encode_var<S, D, false>(d, data, z, out, offset, bits_out, st, boffs_out);
+    return launch_encode_var<S, D, false>(d, data, z, out, offset, bits_out, st, boffs_out, ws);
   if (at) {
     if ((offset & 63) || (z.maxbits & 63))
       return record_error(ZFP_HIP_EINVAL, "blocks at chosen places need word-aligned blocks");
@@ -372,13 +432,27 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
       hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                          d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
     }
-    else if (D < 4 && z.maxbits <= (16u << (2 * D)) && d.g.lax[0] == 0) {
-      // 1-3D, up to 16 bits per value, lanes along x (neighbouring lanes
-      // write neighbouring blocks): the three-wave variant (4D blocks live
-      // in scratch, where a tighter register budget only adds spills)
-      if constexpr (D < 4)
-        hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN, false, 3>), dim3(grid), dim3(BS), 0, st,
-                           (const S*)data, d.g, d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
+    else if (D < 4 && z.maxbits <= 126u * 64u) {
+      // 1-3D: blocks coded in LDS slots, then stored whole; a workgroup's
+      // slots take at most 64 KiB.  Up to 16 bits per value with lanes along
+      // x, the three-wave variant (higher rates need the registers)
+      if constexpr (D < 4) {
+        const u32 nw = z.maxbits >> 6;
+        const u32 stride = (nw + 2) | 1u;
+        unsigned wg = 256;
+        while ((u64)wg * stride * 8 > 65536)
+          wg /= 2;
+        bool order = true; // lane t holds stream block t
+        for (int i = 0; i < d.dims; i++)
+          order = order && d.g.lax[i] == (u32)i;
+        const size_t lds = (size_t)wg * stride * 8;
+        if (z.maxbits <= (16u << (2 * D)) && d.g.lax[0] == 0)
+          hipLaunchKernelGGL((k_encode_slot<S, D, 3>), dim3(grid_for(d.nblocks, wg)), dim3(wg), lds, st,
+                             (const S*)data, d.g, d.nblocks, z, out, offset >> 6, stride, (u32)order);
+        else
+          hipLaunchKernelGGL((k_encode_slot<S, D, 2>), dim3(grid_for(d.nblocks, wg)), dim3(wg), lds, st,
+                             (const S*)data, d.g, d.nblocks, z, out, offset >> 6, stride, (u32)order);
+      }
     }
     else {
       hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED_PLAIN>), dim3(grid), dim3(BS), 0, st, (const S*)data,
@@ -397,17 +471,28 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
 // write pass
 template <typename S, int D, bool REV>
 int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits_out,
-                      hipStream_t st, u64** boffs_out)
+                      hipStream_t st, u64** boffs_out, const Workspace* ws)
 {
   const unsigned BS = 256;
   const unsigned grid = grid_for(d.nblocks, BS);
   u64 nb1 = (u64)d.nblocks + 1;
-  void* lp;
-  int rc = g_lens.get(2 * nb1 * sizeof(u64), &lp);
-  if (rc)
-    return rc;
-  u64* lens = (u64*)lp;
-  u64* offs = lens + nb1;
+  u64* lens;
+  u64* offs;
+  int rc;
+  if (ws) {
+    // the caller's workspace: nothing is allocated, no index is kept
+    lens = ws->lens;
+    offs = ws->offs;
+    boffs_out = nullptr;
+  }
+  else {
+    void* lp;
+    rc = g_lens.get(2 * nb1 * sizeof(u64), &lp);
+    if (rc)
+      return rc;
+    lens = (u64*)lp;
+    offs = lens + nb1;
+  }
   // the caller's copy of the offsets: the scan writes it directly
   struct Held {
     u64* p = nullptr;
@@ -424,11 +509,10 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
     if (index_buffer(held.bytes, &held.p) == ZFP_HIP_OK)
       offs = held.p;
   }
-  HIPCHK(hipMemsetAsync(lens + d.nblocks, 0, sizeof(u64), st));
   hipLaunchKernelGGL((k_encode<S, D, ENC_LENGTH, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                      d.nblocks, z, out, offset, (const u64*)nullptr, lens);
   HIPCHK(hipGetLastError());
-  rc = exclusive_scan(lens, offs, nb1, st);
+  rc = ws ? exclusive_scan_tmp(lens, offs, nb1, ws->tmp, st) : exclusive_scan(lens, offs, nb1, st);
   if (rc)
     return rc;
   bool stream_order = true; // lane t holds stream block t
@@ -443,10 +527,11 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
     const u64 avg_words = bits / (d.nblocks ? d.nblocks : 1) + 1;
     return (u32)std::min<u64>(2048, std::max<u64>(64, 2 * avg_words + 4));
   };
+  // (The workspace variant always waits: it keeps no host-side state.)
   static std::map<std::tuple<u32, u32, u32, int>, u32> last_wcap;
   const auto key = std::make_tuple(z.minbits, z.maxbits, z.maxprec, z.minexp);
   u32 wcap = 0;
-  if (stream_order) {
+  if (stream_order && !ws) {
     auto it = last_wcap.find(key);
     if (it != last_wcap.end())
       wcap = it->second;
@@ -478,7 +563,7 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
   if (!wait)
     HIPCHK(hipMemcpyAsync(&total, offs + d.nblocks, sizeof(u64), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (stream_order) {
+  if (stream_order && !ws) {
     if (last_wcap.size() >= 64)
       last_wcap.clear();
     last_wcap[key] = wcap_for(total);
@@ -592,13 +677,13 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
 
 template <int D>
 int dispatch_encode_d(const Desc& d, const void* data, const Params& z, u64* out, u64 offset,
-                             u64* bits, hipStream_t st, u64** boffs, const u64* at)
+                             u64* bits, hipStream_t st, u64** boffs, const u64* at, const Workspace* ws)
 {
   switch (d.type) {
-    case zfp_type_int32: return launch_encode<i32, D>(d, data, z, out, offset, bits, st, boffs, at);
-    case zfp_type_int64: return launch_encode<i64, D>(d, data, z, out, offset, bits, st, boffs, at);
-    case zfp_type_float: return launch_encode<float, D>(d, data, z, out, offset, bits, st, boffs, at);
-    case zfp_type_double: return launch_encode<double, D>(d, data, z, out, offset, bits, st, boffs, at);
+    case zfp_type_int32: return launch_encode<i32, D>(d, data, z, out, offset, bits, st, boffs, at, ws);
+    case zfp_type_int64: return launch_encode<i64, D>(d, data, z, out, offset, bits, st, boffs, at, ws);
+    case zfp_type_float: return launch_encode<float, D>(d, data, z, out, offset, bits, st, boffs, at, ws);
+    case zfp_type_double: return launch_encode<double, D>(d, data, z, out, offset, bits, st, boffs, at, ws);
   }
   return ZFP_HIP_EINVAL;
 }

@@ -3,6 +3,6 @@
 
 namespace zfphip {
 template int dispatch_encode_d<1>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
-                                    const u64*);
+                                    const u64*, const Workspace*);
 template int dispatch_decode_d<1>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*, hipStream_t);
 } // namespace zfphip

@@ -4,7 +4,7 @@
 
 namespace zfphip {
 template int launch_encode<i32, 4>(const Desc&, const void*, const Params&, u64*, u64, u64*, hipStream_t, u64**,
-                                    const u64*);
+                                    const u64*, const Workspace*);
 template int launch_decode<i32, 4>(const Desc&, void*, const Params&, const u64*, u64, u64, const u64*, u64*,
                                     hipStream_t);
 } // namespace zfphip

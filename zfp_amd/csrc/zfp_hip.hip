@@ -96,13 +96,13 @@ int make_desc(int type, const size_t* n, const ptrdiff_t* s, Desc& d)
 }
 
 int dispatch_encode(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits,
-                    hipStream_t st, u64** boffs, const u64* at = nullptr)
+                    hipStream_t st, u64** boffs, const u64* at = nullptr, const Workspace* ws = nullptr)
 {
   switch (d.dims) {
-    case 1: return dispatch_encode_d<1>(d, data, z, out, offset, bits, st, boffs, at);
-    case 2: return dispatch_encode_d<2>(d, data, z, out, offset, bits, st, boffs, at);
-    case 3: return dispatch_encode_d<3>(d, data, z, out, offset, bits, st, boffs, at);
-    case 4: return dispatch_encode_d<4>(d, data, z, out, offset, bits, st, boffs, at);
+    case 1: return dispatch_encode_d<1>(d, data, z, out, offset, bits, st, boffs, at, ws);
+    case 2: return dispatch_encode_d<2>(d, data, z, out, offset, bits, st, boffs, at, ws);
+    case 3: return dispatch_encode_d<3>(d, data, z, out, offset, bits, st, boffs, at, ws);
+    case 4: return dispatch_encode_d<4>(d, data, z, out, offset, bits, st, boffs, at, ws);
   }
   return ZFP_HIP_EINVAL;
 }
@@ -135,6 +135,7 @@ struct Index {
   size_t n[4] = {0, 0, 0, 0};
   Params z{};
   bool any_params = false;
+  bool host = false;      // keyed by a host buffer (freed and reused by its owner, unseen here)
   u64* d_offsets = nullptr;
   u64 stamp = 0;          // last use, for eviction
 };
@@ -143,7 +144,9 @@ u64 g_index_clock = 0;
 
 // Bounds of the registry: entries beyond them are evicted, least recently
 // used first (a later decode of an evicted stream parses it instead).
-constexpr size_t INDEX_MAX_ENTRIES = 64;
+// Host stream buffers come and go with their owner, so few of them are kept.
+constexpr size_t INDEX_MAX_ENTRIES = 4096;
+constexpr size_t INDEX_MAX_HOST = 64;
 constexpr u64 INDEX_MAX_BYTES = 1ull << 31;
 
 // one index buffer that left the registry, kept for the next variable-rate
@@ -168,19 +171,32 @@ void index_put(const void* key, const Index& ix)
 {
   index_drop(key);
   Index e = ix;
+  e.host = !is_device_ptr(key);
   e.stamp = ++g_index_clock;
   g_index[key] = e;
   for (;;) {
     u64 bytes = 0;
-    auto old = g_index.end();
+    size_t hosts = 0;
+    auto old = g_index.end(), old_host = g_index.end();
     for (auto it = g_index.begin(); it != g_index.end(); ++it) {
       bytes += index_bytes(it->second);
-      if (it->first != key && (old == g_index.end() || it->second.stamp < old->second.stamp))
+      if (it->first == key)
+        continue;
+      if (old == g_index.end() || it->second.stamp < old->second.stamp)
         old = it;
+      if (it->second.host) {
+        hosts++;
+        if (old_host == g_index.end() || it->second.stamp < old_host->second.stamp)
+          old_host = it;
+      }
     }
-    if ((g_index.size() <= INDEX_MAX_ENTRIES && bytes <= INDEX_MAX_BYTES) || old == g_index.end())
+    hosts += e.host ? 1 : 0;
+    if (hosts > INDEX_MAX_HOST && old_host != g_index.end())
+      index_drop(old_host->first);
+    else if ((g_index.size() > INDEX_MAX_ENTRIES || bytes > INDEX_MAX_BYTES) && old != g_index.end())
+      index_drop(old->first);
+    else
       break;
-    index_drop(old->first);
   }
 }
 
