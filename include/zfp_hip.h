@@ -75,6 +75,28 @@ int zfp_hip_encode(int type, const size_t n[4], const ptrdiff_t s[4], const void
                    uint64_t* d_words, uint64_t offset, uint64_t* bits_out,
                    void* hip_stream);
 
+/*
+ * Allocation-free zfp_hip_encode: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * stream and result as zfp_hip_encode, with two differences:
+ *   - the scratch of variable-rate modes (block lengths and offsets, scan
+ *     tile sums) is an 80 MiB device array of the library, held by each GPU
+ *     the library runs on until the process ends; fields of more than
+ *     ZFP_HIP_NOALLOC_MAX_BLOCKS blocks are refused (ZFP_HIP_EINVAL);
+ *   - no block index is recorded, so a later zfp_hip_decode of a
+ *     variable-rate stream locates its blocks from the stream itself.
+ * The call allocates no device memory (fixed-rate encoding uses no scratch
+ * at all).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU race, so serialise them (the library lock does within
+ * one process).
+ */
+#define ZFP_HIP_NOALLOC_MAX_BLOCKS 5000000u
+int zfp_hip_encode_noalloc(int type, const size_t n[4], const ptrdiff_t s[4], const void* d_data,
+                           uint minbits, uint maxbits, uint maxprec, int minexp,
+                           uint64_t* d_words, uint64_t offset, uint64_t* bits_out,
+                           void* hip_stream);
+
 /* Device-level decode, the inverse of zfp_hip_encode.  d_words holds
    nwords readable words (the stream's own ceil((offset + bits) / 64) words
    suffice); no word at or past nwords is read.  *bits_out receives the

@@ -152,6 +152,9 @@ _HIP = {
     "zfp_hip_encode": (ctypes.c_int, [ctypes.c_int, _SZ4, _PD4, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
                                       ctypes.c_uint, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
                                       _P(ctypes.c_uint64), ctypes.c_void_p]),
+    "zfp_hip_encode_noalloc": (ctypes.c_int, [ctypes.c_int, _SZ4, _PD4, ctypes.c_void_p, ctypes.c_uint,
+                                              ctypes.c_uint, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.c_uint64, _P(ctypes.c_uint64), ctypes.c_void_p]),
     "zfp_hip_decode": (ctypes.c_int, [ctypes.c_int, _SZ4, _PD4, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
                                       ctypes.c_uint, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                       _P(ctypes.c_uint64), ctypes.c_void_p]),
