@@ -53,7 +53,15 @@ __device__ __forceinline__ void encode_half(PE& pe, W& w, const PlaneVec& lo, co
     if (wave_any(act)) {
       const int ku = __builtin_amdgcn_readfirstlane(k); // uniform plane number
       const u64 x = (u64)lo[ku] | (N == 64 ? (u64)hi[ku] << 32 : 0ull);
-      if (act)
+      if (wave_all(!act || pe.n >= (u32)N)) {
+        // every coefficient significant in every lane still coding: the
+        // plane verbatim, nothing else
+        if (act) {
+          w.put(x, (u32)N);
+          pe.pos += (u32)N;
+        }
+      }
+      else if (act)
         pe.plane(w, x);
     }
   }

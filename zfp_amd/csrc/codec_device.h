@@ -104,6 +104,8 @@ ZH_INLINE u32 ctz64(u64 x) { return (u32)__builtin_ctzll(x); }
 ZH_INLINE u32 umin(u32 a, u32 b) { return a < b ? a : b; }
 // true when the predicate holds in any active lane of the wave
 ZH_INLINE bool wave_any(bool p) { return __any(p) != 0; }
+// true when the predicate holds in every active lane of the wave
+ZH_INLINE bool wave_all(bool p) { return __any(!p) == 0; }
 
 // 32 words held in VGPRs; an index that is uniform across the wave selects
 // a register without unrolling

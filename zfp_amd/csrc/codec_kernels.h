@@ -105,13 +105,84 @@ ZH_INLINE void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Word runs between the lanes' LDS slots (nw words each, slot stride
+// `stride`) and the stream, when the wave's blocks lie anywhere in the
+// stream (lanes not in stream order): lane i's block starts at stream word
+// start_i.  Each pass moves 64 words -- 64 / nw whole blocks when nw divides
+// 64 -- so every load or store instruction covers whole runs of consecutive
+// words instead of one word of each of 64 distant blocks.  The starts go
+// through LDS (`starts`, 64 words per wave).
+//
+// Both this and the single-run copies below go 8 passes at a time: the
+// eight loads (or LDS reads) are all in flight before the first result is
+// used, so a wave waits out one memory latency per 8 passes, not per pass.
+// `first` (one run) or `starts` (one run per block, in LDS) give the stream
+// word of each (block, word) pair.
+constexpr u32 XFER_BATCH = 8;
+
+template <bool TO_STREAM, bool ONE_RUN>
+ZH_INLINE void wave_copy(u64* __restrict__ stream, u64 nwords, u64* wslots, u32 stride, u32 nw, u32 cnt,
+                         u64 first, const u64* starts)
+{
+  const u32 lane = threadIdx.x & 63u;
+  const u32 total = cnt * nw;
+  u32 blk = lane / nw, wd = lane - blk * nw;
+  const u32 db = 64u / nw, dw = 64u - db * nw;
+  for (u32 q0 = 0; q0 < total; q0 += 64 * XFER_BATCH) {
+    u64 v[XFER_BATCH];
+    u64 gi[XFER_BATCH];
+    u32 li[XFER_BATCH];
+#pragma unroll
+    for (u32 k = 0; k < XFER_BATCH; k++) {
+      // pass k: word wd of block blk
+      li[k] = blk * stride + wd;
+      gi[k] = ONE_RUN ? first + q0 + 64 * k + lane : starts[blk < cnt ? blk : 0] + wd;
+      wd += dw;
+      blk += db;
+      if (wd >= nw) {
+        wd -= nw;
+        blk++;
+      }
+    }
+#pragma unroll
+    for (u32 k = 0; k < XFER_BATCH; k++) {
+      const bool in_run = q0 + 64 * k + lane < total;
+      if constexpr (TO_STREAM)
+        v[k] = in_run ? wslots[li[k]] : 0ull;
+      else
+        v[k] = stream[gi[k] < nwords ? gi[k] : nwords - 1]; // clamped: always a valid address
+    }
+#pragma unroll
+    for (u32 k = 0; k < XFER_BATCH; k++) {
+      const bool in_run = q0 + 64 * k + lane < total;
+      if (in_run) {
+        if constexpr (TO_STREAM)
+          stream[gi[k]] = v[k];
+        else
+          wslots[li[k]] = gi[k] < nwords ? v[k] : 0ull;
+      }
+    }
+  }
+}
+
+template <bool TO_STREAM>
+ZH_INLINE void wave_transfer(u64* __restrict__ stream, u64 nwords, u64* wslots, u32 stride, u32 nw, u32 cnt,
+                             u64 my_start, u64* starts)
+{
+  const u32 lane = threadIdx.x & 63u;
+  starts[lane] = my_start;
+  wave_lds_sync();
+  wave_copy<TO_STREAM, false>(stream, nwords, wslots, stride, nw, cnt, 0, starts);
+  wave_lds_sync();
+}
+
 // Fixed rate with word-aligned blocks (maxbits = 64 * nw, offset a multiple
 // of 64), 1-3D: each lane codes its block into its own slot of LDS (nw words
 // plus two for the last plane's overflow, slot stride `stride`, odd), with
 // the straight-line plane coder; then the block's nw words go to the stream.
 // When the lanes hold consecutive stream blocks (lane order = stream order)
 // the wave's blocks are one run of words, copied with coalesced stores;
-// otherwise each lane stores its own block's words.
+// otherwise the wave stores whole blocks per pass (wave_transfer).
 template <typename S, int D, int OCC>
 __global__ void __launch_bounds__(256, OCC)
 k_encode_slot(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* __restrict__ out, u64 word0,
@@ -140,26 +211,14 @@ k_encode_slot(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* __
     // word0 + t0 * nw .. of the stream; word q of the run is word q % nw of
     // the slot of lane q / nw
     wave_lds_sync();
-    const u32 cnt = umin(64u, nblocks - t0);
-    const u32 total = cnt * nw;
-    const u64* wslots = slots + (u64)(threadIdx.x - lane) * stride;
-    u64* dst = out + word0 + (u64)t0 * nw;
-    u32 blk = lane / nw, wd = lane - blk * nw;
-    const u32 db = 64u / nw, dw = 64u - db * nw;
-    for (u32 q = lane; q < total; q += 64) {
-      dst[q] = wslots[blk * stride + wd];
-      wd += dw;
-      blk += db;
-      if (wd >= nw) {
-        wd -= nw;
-        blk++;
-      }
-    }
+    wave_copy<true, true>(out, 0, slots + (u64)(threadIdx.x - lane) * stride, stride, nw, umin(64u, nblocks - t0),
+                          word0 + (u64)t0 * nw, nullptr);
   }
-  else if (t < nblocks) {
-    u64* dst = out + word0 + (u64)b * nw;
-    for (u32 i = 0; i < nw; i++)
-      dst[i] = slot[i];
+  else {
+    // the wave's blocks lie apart in the stream: whole runs per pass
+    u64* starts = slots + (u64)blockDim.x * stride + (threadIdx.x - lane);
+    wave_transfer<true>(out, 0, slots + (u64)(threadIdx.x - lane) * stride, stride, nw, umin(64u, nblocks - t0),
+                        word0 + (u64)b * nw, starts);
   }
 }
 
@@ -291,14 +350,45 @@ k_encode_waves(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* _
 // start, so when no block disagrees every offset is, by induction, the true
 // start of its block; one disagreement means the index does not belong to
 // this stream and the caller decodes again from a parse.
+//
+// coal (STAGE, word-aligned blocks of nw = maxbits / 64 words, lanes in
+// stream order): the wave's blocks are one run of stream words, which the
+// wave copies into its lanes' slots with coalesced loads.
 template <typename S, int D, bool STAGE, bool REV = false, bool HI_ONLY = false>
 __global__ void __launch_bounds__(256, HI_ONLY ? 3 : 2)
 k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restrict__ in,
          u64 nwords, u64 offset, const u64* __restrict__ boffs, u32 stride,
-         u32* __restrict__ redo = nullptr, u32* __restrict__ nredo = nullptr, u32* __restrict__ bad = nullptr)
+         u32* __restrict__ redo = nullptr, u32* __restrict__ nredo = nullptr, u32* __restrict__ bad = nullptr,
+         u32 coal = 0)
 {
   extern __shared__ u64 stage[];
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if constexpr (STAGE) {
+    if (coal) {
+      const u32 lane = threadIdx.x & 63u;
+      const u32 t0 = t - lane;
+      if (t0 >= nblocks)
+        return; // the whole wave lies past the field
+      const u32 nw = z.maxbits >> 6;
+      const u32 cnt = umin(64u, nblocks - t0);
+      u64* wslots = stage + (u64)(threadIdx.x - lane) * stride;
+      if (coal == 1) {
+        wave_copy<false, true>((u64*)in, nwords, wslots, stride, nw, cnt, (offset >> 6) + (u64)t0 * nw, nullptr);
+        wave_lds_sync();
+      }
+      else {
+        // the wave's blocks lie apart in the stream: whole runs per pass
+        u64 first = 0;
+        if (t < nblocks) {
+          u32 c[4] = {0, 0, 0, 0};
+          const u32 bb = lane_block<D>(t, g, c);
+          first = (offset + (boffs ? boffs[bb] : (u64)bb * z.maxbits)) >> 6;
+        }
+        wave_transfer<false>((u64*)in, nwords, wslots, stride, nw, cnt, first,
+                             stage + (u64)blockDim.x * stride + (threadIdx.x - lane));
+      }
+    }
+  }
   if (t >= nblocks)
     return;
   u32 bc[4] = {0, 0, 0, 0};
@@ -306,24 +396,29 @@ k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restr
   const u64 start = offset + (boffs ? boffs[b] : (u64)b * z.maxbits);
   BitReader r;
   if constexpr (STAGE) {
-    const u64 first = start >> 6;
-    const u32 cnt = (u32)(((start + z.maxbits + 63) >> 6) - first); // <= stride
-    u64* slot = stage + (u64)threadIdx.x * stride;
-    // batches of 16 loads with clamped (always valid) addresses, then the
-    // stores of the words that exist; the host guarantees nwords >= 1
-    for (u32 j0 = 0; j0 < cnt; j0 += 16) {
-      u64 v[16];
-#pragma unroll
-      for (u32 q = 0; q < 16; q++) {
-        const u64 i = first + j0 + q;
-        v[q] = in[i < nwords ? i : nwords - 1];
-      }
-#pragma unroll
-      for (u32 q = 0; q < 16; q++)
-        if (j0 + q < cnt)
-          slot[j0 + q] = first + j0 + q < nwords ? v[q] : 0ull;
+    if (coal) {
+      r.begin(stage + (u64)threadIdx.x * stride, z.maxbits >> 6, 0);
     }
-    r.begin(slot, cnt, start & 63);
+    else {
+      const u64 first = start >> 6;
+      const u32 cnt = (u32)(((start + z.maxbits + 63) >> 6) - first); // <= stride
+      u64* slot = stage + (u64)threadIdx.x * stride;
+      // batches of 16 loads with clamped (always valid) addresses, then the
+      // stores of the words that exist; the host guarantees nwords >= 1
+      for (u32 j0 = 0; j0 < cnt; j0 += 16) {
+        u64 v[16];
+#pragma unroll
+        for (u32 q = 0; q < 16; q++) {
+          const u64 i = first + j0 + q;
+          v[q] = in[i < nwords ? i : nwords - 1];
+        }
+#pragma unroll
+        for (u32 q = 0; q < 16; q++)
+          if (j0 + q < cnt)
+            slot[j0 + q] = first + j0 + q < nwords ? v[q] : 0ull;
+      }
+      r.begin(slot, cnt, start & 63);
+    }
   }
   else {
     r.begin(in, nwords, start);
@@ -432,20 +527,22 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
       hipLaunchKernelGGL((k_encode<S, D, ENC_FIXED>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                          d.nblocks, z, out, offset, (const u64*)nullptr, (u64*)nullptr);
     }
-    else if (D < 4 && z.maxbits <= 126u * 64u) {
+    else if (D < 4 && z.maxbits <= 124u * 64u) {
       // 1-3D: blocks coded in LDS slots, then stored whole; a workgroup's
-      // slots take at most 64 KiB.  Up to 16 bits per value with lanes along
-      // x, the three-wave variant (higher rates need the registers)
+      // slots (and one block start per lane) take at most 64 KiB, which
+      // holds one wave's slots up to 124 words a block.  Up to 16 bits per
+      // value with lanes along x, the three-wave variant (higher rates need
+      // the registers)
       if constexpr (D < 4) {
         const u32 nw = z.maxbits >> 6;
         const u32 stride = (nw + 2) | 1u;
         unsigned wg = 256;
-        while ((u64)wg * stride * 8 > 65536)
+        while (wg > 64 && (u64)wg * (stride + 1) * 8 > 65536)
           wg /= 2;
         bool order = true; // lane t holds stream block t
         for (int i = 0; i < d.dims; i++)
           order = order && d.g.lax[i] == (u32)i;
-        const size_t lds = (size_t)wg * stride * 8;
+        const size_t lds = (size_t)wg * (stride + 1) * 8; // slots, then one block start per lane
         if (z.maxbits <= (16u << (2 * D)) && d.g.lax[0] == 0)
           hipLaunchKernelGGL((k_encode_slot<S, D, 3>), dim3(grid_for(d.nblocks, wg)), dim3(wg), lds, st,
                              (const S*)data, d.g, d.nblocks, z, out, offset >> 6, stride, (u32)order);
@@ -615,10 +712,19 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
   // 64 KiB (an odd slot stride spreads the lanes over the LDS banks)
   const u32 cmax = (u32)((z.maxbits + 63) / 64) + ((offset & 63) || (z.maxbits & 63) ? 1u : 0u);
   const u32 stride = cmax | 1u;
+  // slots, then one block start per lane (the staging of scattered blocks)
   unsigned sbs = 256;
-  while (sbs > 64 && (u64)sbs * stride * 8 > 65536)
+  while (sbs > 64 && (u64)sbs * (stride + 1) * 8 > 65536)
     sbs /= 2;
-  const bool staged = fixed && nwords > 0 && (u64)sbs * stride * 8 <= 65536;
+  const bool staged = fixed && nwords > 0 && (u64)sbs * (stride + 1) * 8 <= 65536;
+  // word-aligned blocks: waves stage their blocks' words with whole-run
+  // loads, one run when the lanes are in stream order (1), else a run per
+  // block (2)
+  bool order = true;
+  for (int i = 0; i < d.dims; i++)
+    order = order && d.g.lax[i] == (u32)i;
+  const bool aligned = !(offset & 63) && !(z.maxbits & 63);
+  const u32 coal = !staged || !aligned ? 0u : order && !boffs ? 1u : 2u;
   if constexpr (std::is_same<S, double>::value && D == 3) {
     // 3D doubles at up to 16 bits per value (the 1D and 2D decoders already
     // run at five or more waves per SIMD): the cast puts each block's largest
@@ -635,8 +741,8 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
       u32* redo = nredo + 1;
       HIPCHK(hipMemsetAsync(nredo, 0, sizeof(u32), st));
       hipLaunchKernelGGL((k_decode<S, D, true, false, true>), dim3(grid_for(d.nblocks, sbs)), dim3(sbs),
-                         (size_t)sbs * stride * 8, st, (S*)data, d.g, d.nblocks, z, in, nwords, offset,
-                         (const u64*)nullptr, stride, redo, nredo);
+                         (size_t)sbs * (stride + 1) * 8, st, (S*)data, d.g, d.nblocks, z, in, nwords, offset,
+                         (const u64*)nullptr, stride, redo, nredo, (u32*)nullptr, coal);
       // 512 workgroups fill the chip at two waves per SIMD
       hipLaunchKernelGGL((k_decode_list<S, D>), dim3(std::min(grid, 512u)), dim3(BS), 0, st, (S*)data, d.g, z,
                          in, nwords, offset, (const u32*)redo, (const u32*)nredo);
@@ -649,8 +755,8 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
   if (staged) {
     // fixed rate: boffs, when given, places block b at offset + boffs[b]
     hipLaunchKernelGGL((k_decode<S, D, true>), dim3(grid_for(d.nblocks, sbs)), dim3(sbs),
-                       (size_t)sbs * stride * 8, st, (S*)data, d.g, d.nblocks, z, in, nwords, offset,
-                       boffs, stride);
+                       (size_t)sbs * (stride + 1) * 8, st, (S*)data, d.g, d.nblocks, z, in, nwords, offset,
+                       boffs, stride, (u32*)nullptr, (u32*)nullptr, (u32*)nullptr, coal);
   }
   else if (rev) {
     hipLaunchKernelGGL((k_decode<S, D, false, true>), dim3(grid), dim3(BS), 0, st, (S*)data, d.g, d.nblocks, z,
