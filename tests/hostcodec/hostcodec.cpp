@@ -141,6 +141,32 @@ uint64_t enc_slot(const void* data, const size_t* n, const ptrdiff_t* s, const P
   return (uint64_t)nb * z.maxbits;
 }
 
+// fixed rate through the slot reader: each block's words copied into a slot
+// (plus one word after them), read by bit index; blocks at b * maxbits from
+// bit `offset`
+template <typename S, int D>
+uint64_t dec_slot(void* data, const size_t* n, const ptrdiff_t* s, const Params& z, const uint64_t* words,
+                  uint64_t nwords, uint64_t offset)
+{
+  Geom g;
+  u32 nb;
+  make_geom(D, n, s, g, nb);
+  std::vector<uint64_t> slot(z.maxbits / 64 + 3);
+  for (u32 b = 0; b < nb; b++) {
+    u32 bc[4] = {0, 0, 0, 0};
+    block_coords<D>(b, g, bc);
+    const uint64_t start = offset + (uint64_t)b * z.maxbits;
+    for (size_t i = 0; i < slot.size(); i++) {
+      const uint64_t w = start / 64 + i;
+      slot[i] = w < nwords ? words[w] : 0;
+    }
+    SlotReader r;
+    r.begin(slot.data(), (u32)(start & 63));
+    decode_block<S, D, true>(r, (S*)data, g, bc, z);
+  }
+  return (uint64_t)nb * z.maxbits;
+}
+
 template <typename S>
 uint64_t enc_t(int dims, const void* data, const size_t* n, const ptrdiff_t* s, const Params& z,
                uint64_t* words, uint64_t offset)
@@ -215,6 +241,28 @@ extern "C" uint64_t hostcodec_compress_slot(int type, const size_t* n, const ptr
     default: ZH_SLOT(double);
   }
 #undef ZH_SLOT
+}
+
+// Fixed-rate 1-4D fields through the slot reader (the decoder of blocks
+// staged in LDS); returns the bits consumed.
+extern "C" uint64_t hostcodec_decompress_slot(int type, const size_t* n, const ptrdiff_t* s, void* data,
+                                              unsigned maxbits, unsigned maxprec, int minexp, const uint64_t* words,
+                                              uint64_t nwords, uint64_t offset)
+{
+  int dims = n[0] ? n[1] ? n[2] ? n[3] ? 4 : 3 : 2 : 1 : 0;
+  Params z{maxbits, maxbits, maxprec, minexp};
+#define ZH_DSLOT(S) \
+  return dims == 1 ? dec_slot<S, 1>(data, n, s, z, words, nwords, offset) \
+       : dims == 2 ? dec_slot<S, 2>(data, n, s, z, words, nwords, offset) \
+       : dims == 3 ? dec_slot<S, 3>(data, n, s, z, words, nwords, offset) \
+                   : dec_slot<S, 4>(data, n, s, z, words, nwords, offset)
+  switch (type) {
+    case 1: ZH_DSLOT(i32);
+    case 2: ZH_DSLOT(i64);
+    case 3: ZH_DSLOT(float);
+    default: ZH_DSLOT(double);
+  }
+#undef ZH_DSLOT
 }
 
 // Fixed-rate int64/double fields of 1-3 dimensions through the two-pass

@@ -179,3 +179,34 @@ def test_slot_encoder_matches_oracle(shape, tname, rate):
                hw.ctypes.data_as(u64p), offset)
         assert hb == eb
         assert np.array_equal(hw, ew)
+
+
+@pytest.mark.parametrize("rate", [1, 4, 8, 16, 21.5, 32])
+@pytest.mark.parametrize("tname", list(TYPES))
+@pytest.mark.parametrize("shape", list(SHAPES))
+def test_slot_reader_decoder_matches_oracle(shape, tname, rate):
+    """The decoder of fixed-rate blocks staged in LDS (bit-index reads from
+    the slot, no refill window) reconstructs serial zfp's values, at word-
+    aligned and unaligned block offsets."""
+    n = SHAPES[shape]
+    t = TYPES[tname]
+    p = stream_params_rate(rate, t, len(n))
+    a = smooth_field(n, t, seed=zlib.crc32(f"sread/{shape}/{tname}".encode()) & 0xffff,
+                     special=0.02 if t in (T_FLOAT, T_DOUBLE) else 0.0)
+    f = lib().hostcodec_decompress_slot
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_int, szp, pdp, ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_int, u64p,
+                  ctypes.c_uint64, ctypes.c_uint64]
+    for layout, offset in (("as_is", 0), ("permuted_neg", 77)):
+        if layout != "as_is" and len(n) == 1:
+            continue
+        base, off, s = make_layout(a, n, layout)
+        ew, eb = oracle_compress(base, off, n, s, t, p, offset=offset)
+        ref = np.zeros_like(base)
+        oracle_decompress(ew, ref, off, n, s, t, p, offset=offset)
+        out = np.zeros_like(base)
+        nn, ss = field_args(n, s)
+        bits = f(t, nn, ss, ctypes.c_void_p(out.ctypes.data + off * base.itemsize), p[1], p[2], p[3],
+                 ew.ctypes.data_as(u64p), ew.size, offset)
+        assert bits == eb
+        assert np.array_equal(out.view(np.uint8), ref.view(np.uint8))

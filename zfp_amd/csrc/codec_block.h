@@ -89,8 +89,8 @@ __device__ __forceinline__ u32 encode_planes(W& w, const UInt* u, u32 maxbits, u
 
 // decoding mirror (src/template/decode.c:78-278); u receives the
 // coefficients' bits, zero where no plane was coded
-template <int N>
-__device__ __forceinline__ void decode_half(PlaneDecoder<N>& pd, BitReader& r, PlaneVec& lo, PlaneVec& hi,
+template <int N, typename R>
+__device__ __forceinline__ void decode_half(PlaneDecoder<N>& pd, R& r, PlaneVec& lo, PlaneVec& hi,
                                             u32 base)
 {
   for (int k = 31; k >= 0; k--) {
@@ -112,8 +112,8 @@ __device__ __forceinline__ void decode_half(PlaneDecoder<N>& pd, BitReader& r, P
 // when the block goes on below plane 32; such a block's coefficients lack
 // their low half and it has to be decoded again in full.  Leaving the low
 // half out frees the registers that hold it.
-template <int N, int P, typename UInt, bool STORE, bool HI_ONLY = false>
-__device__ __forceinline__ u32 decode_planes(BitReader& r, UInt* u, u32 maxbits, u32 maxprec,
+template <int N, int P, typename UInt, bool STORE, bool HI_ONLY = false, typename R = BitReader>
+__device__ __forceinline__ u32 decode_planes(R& r, UInt* u, u32 maxbits, u32 maxprec,
                                              bool* more = nullptr)
 {
   static_assert(!HI_ONLY || P == 64, "HI_ONLY splits 64-bit coefficients");
@@ -232,8 +232,8 @@ __device__ __forceinline__ u32 encode_block4(W& w, const S* data, const Geom& g,
   return hdr + c;
 }
 
-template <typename S, bool STORE>
-__device__ __forceinline__ u32 decode_block4(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z)
+template <typename S, bool STORE, typename R = BitReader>
+__device__ __forceinline__ u32 decode_block4(R& r, S* data, const Geom& g, const u32* bc, const Params& z)
 {
   typedef Traits<S> T;
   typedef typename T::Int Int;
@@ -364,8 +364,8 @@ __device__ __forceinline__ u32 encode_block(W& w, const S* data, const Geom& g, 
 // decode one block (src/template/decodef.c:6-44, decodei.c:6-10,
 // decode.c:281-298); returns the block's length in bits
 // ---------------------------------------------------------------------------
-template <typename S, int D, bool STORE, bool HI_ONLY = false>
-__device__ __forceinline__ u32 decode_block(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z,
+template <typename S, int D, bool STORE, bool HI_ONLY = false, typename R = BitReader>
+__device__ __forceinline__ u32 decode_block(R& r, S* data, const Geom& g, const u32* bc, const Params& z,
                                             bool* more = nullptr)
 {
   if constexpr (D == 4)
@@ -564,8 +564,8 @@ __device__ __forceinline__ u32 encode_block_rev(W& w, const S* data, const Geom&
   }
 }
 
-template <typename S, int D, bool STORE>
-__device__ __forceinline__ u32 rev_decode_ints(BitReader& r, typename Traits<S>::Int* q, u32 minbits,
+template <typename S, int D, bool STORE, typename R = BitReader>
+__device__ __forceinline__ u32 rev_decode_ints(R& r, typename Traits<S>::Int* q, u32 minbits,
                                                u32 maxbits)
 {
   typedef Traits<S> T;
@@ -590,8 +590,8 @@ __device__ __forceinline__ u32 rev_decode_ints(BitReader& r, typename Traits<S>:
   return c;
 }
 
-template <typename S, int D, bool STORE>
-__device__ __forceinline__ u32 decode_block_rev(BitReader& r, S* data, const Geom& g, const u32* bc,
+template <typename S, int D, bool STORE, typename R = BitReader>
+__device__ __forceinline__ u32 decode_block_rev(R& r, S* data, const Geom& g, const u32* bc,
                                                 const Params& z)
 {
   typedef Traits<S> T;

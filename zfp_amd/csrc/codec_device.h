@@ -747,6 +747,26 @@ struct BitReader {
   }
 };
 
+// Reader of one block staged in LDS: the position is a bit index into the
+// slot, and every peek reads the two words it spans, so reading keeps no
+// window to shift and refill.  The word after the block's last must be
+// readable (its bits are past the block and are masked off by the coder).
+struct SlotReader {
+  const u64* slot;
+  u32 pos;
+
+  ZH_INLINE void begin(const u64* s, u32 bit) { slot = s; pos = bit; }
+  ZH_INLINE u64 peek64() const
+  {
+    const u32 i = pos >> 6, s = pos & 63u;
+    return (slot[i] >> s) | ((slot[i + 1] << 1) << (63 - s));
+  }
+  ZH_INLINE u64 peek(u32 n) const { return peek64() & lowmask(n); }
+  ZH_INLINE void skip(u32 n) { pos += n; }
+  ZH_INLINE u64 read(u32 n) { u64 v = peek(n); skip(n); return v; }
+  ZH_INLINE void skip_long(u64 n) { pos += (u32)n; }
+};
+
 // --------------------------------------------------------------------------
 // embedded coder for N <= 64 coefficients, one bit plane at a time
 // (src/template/encode.c:90-256).  Plane k of the coefficients is the word
@@ -955,7 +975,8 @@ struct PlaneDecoder {
   ZH_INLINE u32 used() const { return budget0 - budget; }
 
   // one plane; the caller checks wants(k)
-  ZH_INLINE u64 plane(BitReader& r)
+  template <typename R>
+  ZH_INLINE u64 plane(R& r)
   {
     // verbatim bits of the significant coefficients; when the group test
     // that follows them is a 0 (the usual end of a plane) it is taken in
@@ -1145,8 +1166,8 @@ ZH_INLINE u32 encode_planes256(W& w, const UInt* u, u32 maxbits, u32 maxprec)
 
 // the decoder into a plane table (planes_chunk layout); planes that are not
 // coded stay zero.  Returns the bits consumed.
-template <int P>
-ZH_INLINE u32 decode_table256(BitReader& r, u64* X, u32 maxbits, u32 maxprec)
+template <int P, typename R>
+ZH_INLINE u32 decode_table256(R& r, u64* X, u32 maxbits, u32 maxprec)
 {
   const u32 N = 256;
   const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
@@ -1200,8 +1221,8 @@ ZH_INLINE u32 decode_table256(BitReader& r, u64* X, u32 maxbits, u32 maxprec)
   return budget0 - budget;
 }
 
-template <typename UInt, int P>
-ZH_INLINE u32 decode_planes256(BitReader& r, UInt* u, u32 maxbits, u32 maxprec)
+template <typename UInt, int P, typename R>
+ZH_INLINE u32 decode_planes256(R& r, UInt* u, u32 maxbits, u32 maxprec)
 {
   u64 X[4 * P];
   const u32 c = decode_table256<P>(r, X, maxbits, maxprec);

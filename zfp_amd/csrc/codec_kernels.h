@@ -48,8 +48,8 @@ __device__ __forceinline__ u32 encode_any(W& w, const S* data, const Geom& g, co
     return encode_block<S, D>(w, data, g, bc, z);
 }
 
-template <typename S, int D, bool STORE, bool REV>
-__device__ __forceinline__ u32 decode_any(BitReader& r, S* data, const Geom& g, const u32* bc, const Params& z)
+template <typename S, int D, bool STORE, bool REV, typename R>
+__device__ __forceinline__ u32 decode_any(R& r, S* data, const Geom& g, const u32* bc, const Params& z)
 {
   if constexpr (REV)
     return decode_block_rev<S, D, STORE>(r, data, g, bc, z);
@@ -333,6 +333,11 @@ k_encode_waves(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* _
   }
 }
 
+template <typename S, int D, bool REV, bool HI_ONLY, typename R>
+__device__ __forceinline__ void decode_lane(R& r, S* __restrict__ data, const Geom& g, const u32* bc,
+                                            const Params& z, u32 t, u32 b, const u64* __restrict__ boffs,
+                                            u32* __restrict__ redo, u32* __restrict__ nredo, u32* __restrict__ bad);
+
 // STAGE (fixed rate, blocks of at most `stride` words): each lane first
 // copies its block's words into its own slot of LDS, with all loads in
 // flight at once, and the bit reader then refills from LDS.  Reading the
@@ -394,15 +399,12 @@ k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restr
   u32 bc[4] = {0, 0, 0, 0};
   const u32 b = lane_block<D>(t, g, bc);
   const u64 start = offset + (boffs ? boffs[b] : (u64)b * z.maxbits);
-  BitReader r;
   if constexpr (STAGE) {
-    if (coal) {
-      r.begin(stage + (u64)threadIdx.x * stride, z.maxbits >> 6, 0);
-    }
-    else {
+    u64* slot = stage + (u64)threadIdx.x * stride;
+    u32 bit0 = 0;
+    if (!coal) {
       const u64 first = start >> 6;
       const u32 cnt = (u32)(((start + z.maxbits + 63) >> 6) - first); // <= stride
-      u64* slot = stage + (u64)threadIdx.x * stride;
       // batches of 16 loads with clamped (always valid) addresses, then the
       // stores of the words that exist; the host guarantees nwords >= 1
       for (u32 j0 = 0; j0 < cnt; j0 += 16) {
@@ -417,12 +419,26 @@ k_decode(S* __restrict__ data, Geom g, u32 nblocks, Params z, const u64* __restr
           if (j0 + q < cnt)
             slot[j0 + q] = first + j0 + q < nwords ? v[q] : 0ull;
       }
-      r.begin(slot, cnt, start & 63);
+      bit0 = (u32)(start & 63);
     }
+    // the slot's stride leaves a readable word after the block's last
+    SlotReader r;
+    r.begin(slot, bit0);
+    decode_lane<S, D, REV, HI_ONLY>(r, data, g, bc, z, t, b, boffs, redo, nredo, bad);
   }
   else {
+    BitReader r;
     r.begin(in, nwords, start);
+    decode_lane<S, D, REV, HI_ONLY>(r, data, g, bc, z, t, b, boffs, redo, nredo, bad);
   }
+}
+
+// the decode of one lane's block, and the bookkeeping of k_decode's modes
+template <typename S, int D, bool REV, bool HI_ONLY, typename R>
+__device__ __forceinline__ void decode_lane(R& r, S* __restrict__ data, const Geom& g, const u32* bc,
+                                            const Params& z, u32 t, u32 b, const u64* __restrict__ boffs,
+                                            u32* __restrict__ redo, u32* __restrict__ nredo, u32* __restrict__ bad)
+{
   if constexpr (HI_ONLY) {
     static_assert(!REV && Traits<S>::P == 64 && D < 4, "HI_ONLY: fixed-rate 64-bit types, 1-3D");
     bool more = false;
