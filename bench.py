@@ -706,7 +706,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              f"(sharding a field across GPUs is not built)", file=sys.stderr)
         raise SystemExit(2)
     value, ms_step, roofline, detail, checks, f = gpu_bench(args)
     cpu = None if args.no_cpu else cpu_baseline(args, f)

@@ -32,6 +32,18 @@ bool make_geom(int dims, const size_t* n, const ptrdiff_t* s, Geom& g, u32& nblo
     }
   }
   g.dims = dims;
+  // rows run along the axis of smallest |stride|, as the library sets it
+  for (int i = 0; i < 4; i++)
+    g.lax[i] = (u32)i;
+  for (int i = 1; i < dims; i++)
+    for (int j = i; j > 0; j--) {
+      int64_t a = g.s[g.lax[j]], b = g.s[g.lax[j - 1]];
+      if ((a < 0 ? -a : a) < (b < 0 ? -b : b)) {
+        u32 t = g.lax[j]; g.lax[j] = g.lax[j - 1]; g.lax[j - 1] = t;
+      }
+    }
+  for (int i = 0; i < 4; i++)
+    g.lnb[i] = g.nb[g.lax[i]];
   nblocks = (u32)nb;
   return true;
 }

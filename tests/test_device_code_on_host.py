@@ -38,7 +38,9 @@ def lib():
 
 
 TYPES = {"i32": T_INT32, "i64": T_INT64, "f32": T_FLOAT, "f64": T_DOUBLE}
-SHAPES = {"1d": [103], "2d": [13, 11], "3d": [9, 10, 7], "4d": [5, 6, 7, 5], "3d_full": [16, 16, 8]}
+SHAPES = {"1d": [103], "2d": [13, 11], "3d": [9, 10, 7], "4d": [5, 6, 7, 5], "3d_full": [16, 16, 8],
+          # even extents: 16-byte rows along the unit-stride axis next to partial blocks
+          "2d_edge": [10, 6], "3d_edge": [6, 12, 10]}
 MODES = {
     "rate8": lambda t, d: stream_params_rate(8, t, d),
     "rate21": lambda t, d: stream_params_rate(21.5, t, d),

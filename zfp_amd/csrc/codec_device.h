@@ -171,10 +171,85 @@ ZH_INLINE void axis_offsets(i64 s, u32 m, i64* o)
   o[3] = m >= 4 ? 3 * s : 0;
 }
 
+// A row of a block is its four values along one axis.  Rows along an axis of
+// stride 1 that start on a 16-byte boundary move as 16-byte vectors: one
+// access per row for 4-byte scalars, two for 8-byte ones.
+template <typename S> struct Row16;
+template <> struct Row16<float> { typedef float T __attribute__((ext_vector_type(4))); };
+template <> struct Row16<i32> { typedef i32 T __attribute__((ext_vector_type(4))); };
+template <> struct Row16<double> { typedef double T __attribute__((ext_vector_type(2))); };
+template <> struct Row16<i64> { typedef i64 T __attribute__((ext_vector_type(2))); };
+
+// Whether every row of the blocks along axis a can move as 16-byte vectors:
+// a has stride 1, the field starts on a 16-byte boundary, and the other
+// strides keep row starts on one (a row starts at a multiple of 4 along a).
+template <typename S, int D>
+ZH_INLINE bool rows16(const void* data, const Geom& g, int a)
+{
+  bool ok = g.s[a] == 1 && ((uintptr_t)data & 15) == 0;
+#pragma unroll
+  for (int d = 0; d < D; d++)
+    ok = ok && (d == a || ((u64)g.s[d] * sizeof(S)) % 16 == 0);
+  return ok;
+}
+
+// The index in v[] (x fastest) of value i of row r of a block whose rows run
+// along axis A; r numbers the other axes' positions, lowest axis fastest
+template <int D, int A>
+ZH_INLINE constexpr int row_value(int r, int i)
+{
+  int k = 0, sh = 0;
+  for (int d = 0; d < D; d++) {
+    if (d == A)
+      k += i << (2 * d);
+    else {
+      k += ((r >> sh) & 3) << (2 * d);
+      sh += 2;
+    }
+  }
+  return k;
+}
+
+// The block's rows along axis A as 16-byte vectors (whole block along A,
+// rows16 holds): the values of gather<S,D>, loaded in memory order
+template <typename S, int D, int A>
+ZH_INLINE void gather_rows16(S* v, const S* p, const i64 (*o)[4])
+{
+  constexpr int N = 1 << (2 * D);
+  typedef typename Row16<S>::T V;
+  constexpr int L = 16 / sizeof(S);
+#pragma unroll
+  for (int r = 0; r < N / 4; r++) {
+    const S* q = p;
+#pragma unroll
+    for (int d = 0, sh = 0; d < D; d++)
+      if (d != A) { q += o[d][(r >> sh) & 3]; sh += 2; }
+#pragma unroll
+    for (int h = 0; h < 4 / L; h++) {
+      const V x = *(const V*)(q + h * L);
+#pragma unroll
+      for (int i = 0; i < L; i++)
+        v[row_value<D, A>(r, h * L + i)] = x[i];
+    }
+  }
+}
+
+// The axis whose rows may move as 16-byte vectors for this lane's block, or
+// -1: the lane axis lax[0] (smallest |stride|) when rows16 holds for it and
+// the block is whole along it; x only, for 4D blocks
+template <typename S, int D>
+ZH_INLINE int row_axis(const void* data, const Geom& g, const u32* b)
+{
+  const int a = D > 1 && g.lax[0] < (u32)D ? (int)g.lax[0] : 0;
+  return (D < 4 || a == 0) && rows16<S, D>(data, g, a) && g.n[a] - 4 * b[a] >= 4 ? a : -1;
+}
+
 // Gather the 4^D values of block (bx,by,bz,bw) into v[] with partial-block
 // padding (encode1.c:4-20, encode2.c:4-25, encode3.c:4-31, encode4.c:4-38).
 // Whole and partial blocks take the same path: padding only changes the
-// offset tables.
+// offset tables.  Where row_axis allows, a lane loads its rows along the
+// axis of unit stride as vectors, in memory order; the order of the loads
+// does not change the values.
 template <typename S, int D>
 ZH_INLINE void gather(S* v, const S* data, const Geom& g, const u32* b)
 {
@@ -187,19 +262,64 @@ ZH_INLINE void gather(S* v, const S* data, const Geom& g, const u32* b)
     p += (i64)c * g.s[d];
     axis_offsets(g.s[d], umin(4u, g.n[d] - c), o[d]);
   }
+  const int a = row_axis<S, D>(data, g, b);
+  if (D > 1 && D < 4 && a == 1)
+    gather_rows16<S, D, (D > 1 ? 1 : 0)>(v, p, o);
+  else if (D > 2 && D < 4 && a == 2)
+    gather_rows16<S, D, (D > 2 ? 2 : 0)>(v, p, o);
+  else if (a == 0)
+    gather_rows16<S, D, 0>(v, p, o);
+  else {
+#pragma unroll
+    for (int r = 0; r < N / 4; r++) {
+      const S* q = p;
+      if (D > 1) q += o[1][r & 3];
+      if (D > 2) q += o[2][(r >> 2) & 3];
+      if (D > 3) q += o[3][r >> 4];
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+        v[4 * r + x] = q[o[0][x]];
+    }
+  }
+}
+
+// The block's rows along axis A as 16-byte vectors (the counterpart of
+// gather_rows16; rows outside the array along the other axes are skipped)
+template <typename S, int D, int A>
+ZH_INLINE void scatter_rows16(const S* v, S* p, const Geom& g, const u32* m)
+{
+  constexpr int N = 1 << (2 * D);
+  typedef typename Row16<S>::T V;
+  constexpr int L = 16 / sizeof(S);
 #pragma unroll
   for (int r = 0; r < N / 4; r++) {
-    const S* q = p;
-    if (D > 1) q += o[1][r & 3];
-    if (D > 2) q += o[2][(r >> 2) & 3];
-    if (D > 3) q += o[3][r >> 4];
+    bool ok = true;
+    S* q = p;
 #pragma unroll
-    for (int x = 0; x < 4; x++)
-      v[4 * r + x] = q[o[0][x]];
+    for (int d = 0, sh = 0; d < D; d++)
+      if (d != A) {
+        const u32 c = (r >> sh) & 3;
+        ok = ok && c < m[d];
+        q += (i64)c * g.s[d];
+        sh += 2;
+      }
+    if (ok) {
+#pragma unroll
+      for (int h = 0; h < 4 / L; h++) {
+        V x;
+#pragma unroll
+        for (int i = 0; i < L; i++)
+          x[i] = v[row_value<D, A>(r, h * L + i)];
+        *(V*)(q + h * L) = x;
+      }
+    }
   }
 }
 
 // Write back only the valid part of a block (decode1.c:4-19 ... decode4.c:4-25).
+// Where row_axis allows, a lane stores whole rows along the axis of unit
+// stride as vectors, filling cache-line segments in consecutive stores
+// whatever the axis order of the view.
 template <typename S, int D>
 ZH_INLINE void scatter(const S* v, S* data, const Geom& g, const u32* b)
 {
@@ -212,18 +332,27 @@ ZH_INLINE void scatter(const S* v, S* data, const Geom& g, const u32* b)
     m[d] = umin(4u, g.n[d] - o);
     p += (i64)o * g.s[d];
   }
+  const int a = row_axis<S, D>(data, g, b);
+  if (D > 1 && D < 4 && a == 1)
+    scatter_rows16<S, D, (D > 1 ? 1 : 0)>(v, p, g, m);
+  else if (D > 2 && D < 4 && a == 2)
+    scatter_rows16<S, D, (D > 2 ? 2 : 0)>(v, p, g, m);
+  else if (a == 0)
+    scatter_rows16<S, D, 0>(v, p, g, m);
+  else {
 #pragma unroll
-  for (int r = 0; r < N / 4; r++) {
-    const u32 y = r & 3, z = (r >> 2) & 3, w = r >> 4;
-    bool ok = true;
-    S* q = p;
-    if (D > 1) { ok = ok && y < m[1]; q += (i64)y * g.s[1]; }
-    if (D > 2) { ok = ok && z < m[2]; q += (i64)z * g.s[2]; }
-    if (D > 3) { ok = ok && w < m[3]; q += (i64)w * g.s[3]; }
+    for (int r = 0; r < N / 4; r++) {
+      const u32 y = r & 3, z = (r >> 2) & 3, w = r >> 4;
+      bool ok = true;
+      S* q = p;
+      if (D > 1) { ok = ok && y < m[1]; q += (i64)y * g.s[1]; }
+      if (D > 2) { ok = ok && z < m[2]; q += (i64)z * g.s[2]; }
+      if (D > 3) { ok = ok && w < m[3]; q += (i64)w * g.s[3]; }
 #pragma unroll
-    for (u32 x = 0; x < 4; x++)
-      if (ok && x < m[0])
-        q[(i64)x * g.s[0]] = v[4 * r + x];
+      for (u32 x = 0; x < 4; x++)
+        if (ok && x < m[0])
+          q[(i64)x * g.s[0]] = v[4 * r + x];
+    }
   }
 }
 
