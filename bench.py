@@ -115,8 +115,8 @@ def pass_kernels(op, rate, layout):
     launch_decode for 3D doubles at a word-aligned fixed rate)."""
     if op == "compress":
         if rate <= 16 and layout == "contiguous":
-            return "k_encode<double,3,ENC_FIXED_PLAIN,false,3>"
-        return "k_encode<double,3,ENC_FIXED_PLAIN,false,2>"
+            return "k_encode_slot<double,3,3>"
+        return "k_encode_slot<double,3,2>"
     if rate <= 16:
         return "k_decode<double,3,STAGE,false,HI_ONLY> + k_decode_list<double,3>"
     return "k_decode<double,3,STAGE>"
