@@ -296,3 +296,65 @@ extern "C" uint64_t hostcodec_decompress_split(int type, const size_t* n, const 
        : dims == 2 ? dec_split<double, 2>(data, n, s, z, words, nwords, offset, nredo)
                    : dec_split<double, 3>(data, n, s, z, words, nwords, offset, nredo);
 }
+
+// The chain rounds of the parallel index recovery (chain_lane, the body of
+// k_chain), run lane after lane on the host with the launcher's schedule:
+// phase 0, rounds of phase 1 until no exit changes, the scan of the counts,
+// phase 2.  Segments are L bits (a multiple of 64).  Returns the number of
+// phase-1 rounds, or -1 when the stream's bits hold fewer than nblocks
+// blocks; boffs receives nblocks + 1 offsets relative to `offset`.
+namespace {
+template <typename S, int D, bool REV>
+int chain_t(const size_t* n, const ptrdiff_t* s, const Params& z, const uint64_t* words, uint64_t nwords,
+            uint64_t offset, uint64_t L, uint64_t* boffs)
+{
+  Geom g;
+  u32 nb;
+  make_geom(D, n, s, g, nb);
+  const uint64_t span = nwords * 64 - offset;
+  const u32 P = (u32)((span + L - 1) / L);
+  std::vector<uint64_t> start(P), ex0(P), ex1(P), cnt(P), base(P);
+  u32 changed = 0;
+  ChainArgs a;
+  a.g = g; a.z = z; a.in = words; a.nwords = nwords; a.offset = offset; a.span = span; a.L = L;
+  a.start = start.data(); a.cnt = cnt.data(); a.base = base.data(); a.boffs = boffs; a.changed = &changed;
+  a.P = P; a.nblocks = nb;
+  uint64_t* ex[2] = {ex0.data(), ex1.data()};
+  a.exit_prev = ex[1]; a.exit_new = ex[0];
+  for (u32 i = 0; i < P; i++) chain_lane<S, D, REV>(a, i, 0);
+  int cur = 0, rounds = 0;
+  for (;; rounds++) {
+    changed = 0;
+    a.exit_prev = ex[cur]; a.exit_new = ex[cur ^ 1];
+    for (u32 i = 0; i < P; i++) chain_lane<S, D, REV>(a, i, 1);
+    cur ^= 1;
+    if (!changed) break;
+  }
+  uint64_t total = 0;
+  for (u32 i = 0; i < P; i++) { base[i] = total; total += cnt[i]; }
+  if (total < nb) return -1;
+  for (u32 i = 0; i < P; i++) chain_lane<S, D, REV>(a, i, 2);
+  return rounds + 1;
+}
+} // namespace
+
+extern "C" int hostcodec_chain_offsets(int type, const size_t* n, const ptrdiff_t* s, unsigned minbits,
+                                       unsigned maxbits, unsigned maxprec, int minexp, const uint64_t* words,
+                                       uint64_t nwords, uint64_t offset, uint64_t L, uint64_t* boffs)
+{
+  int dims = n[0] ? n[1] ? n[2] ? n[3] ? 4 : 3 : 2 : 1 : 0;
+  Params z{minbits, maxbits, maxprec, minexp};
+  const bool rev = minexp < -1074;
+#define ZH_CHAIN(S) \
+  return dims == 1 ? (rev ? chain_t<S, 1, true> : chain_t<S, 1, false>)(n, s, z, words, nwords, offset, L, boffs) \
+       : dims == 2 ? (rev ? chain_t<S, 2, true> : chain_t<S, 2, false>)(n, s, z, words, nwords, offset, L, boffs) \
+       : dims == 3 ? (rev ? chain_t<S, 3, true> : chain_t<S, 3, false>)(n, s, z, words, nwords, offset, L, boffs) \
+                   : (rev ? chain_t<S, 4, true> : chain_t<S, 4, false>)(n, s, z, words, nwords, offset, L, boffs)
+  switch (type) {
+    case 1: ZH_CHAIN(i32);
+    case 2: ZH_CHAIN(i64);
+    case 3: ZH_CHAIN(float);
+    default: ZH_CHAIN(double);
+  }
+#undef ZH_CHAIN
+}

@@ -212,3 +212,53 @@ def test_slot_reader_decoder_matches_oracle(shape, tname, rate):
                  ew.ctypes.data_as(u64p), ew.size, offset)
         assert bits == eb
         assert np.array_equal(out.view(np.uint8), ref.view(np.uint8))
+
+
+CHAIN_MODES = ["prec16", "prec28", "acc1e-3", "acc1e-7", "expert", "reversible", "rev_expert"]
+CHAIN_SHAPES = {"1d": [3001], "2d": [61, 67], "3d": [21, 18, 23], "4d": [9, 10, 7, 11]}
+
+
+@pytest.mark.parametrize("mode", CHAIN_MODES)
+@pytest.mark.parametrize("tname", list(TYPES))
+@pytest.mark.parametrize("shape", list(CHAIN_SHAPES))
+def test_chain_offsets_match_serial_parse(shape, tname, mode):
+    """The parallel recovery of block offsets (chain_lane, the body of
+    k_chain, run lane by lane with the launcher's rounds) gives exactly the
+    block starts of serial zfp's stream, for segments much shorter than the
+    distance at which chains started at arbitrary bits meet the true one (so
+    that many rounds are needed), at bit offsets 0 and 77, with arbitrary
+    words after the stream, and refuses a stream cut short."""
+    n = CHAIN_SHAPES[shape]
+    t = TYPES[tname]
+    p = MODES[mode](t, len(n))
+    a = smooth_field(n, t, seed=zlib.crc32(f"chain/{shape}/{tname}/{mode}".encode()) & 0xffff,
+                     special=0.02 if t in (T_FLOAT, T_DOUBLE) else 0.0)
+    f = lib().hostcodec_chain_offsets
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, szp, pdp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_int, u64p,
+                  ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, u64p]
+    nn, ss = field_args(n, None)
+    rng = np.random.default_rng(zlib.crc32(f"{shape}{tname}{mode}".encode()))
+    for offset in (0, 77):
+        words, bits, bb = oracle_compress(a, 0, n, None, t, p, offset=offset, block_bits=True)
+        truth = np.concatenate([[0], np.cumsum(bb.astype(np.uint64))]).astype(np.uint64)
+        assert truth[-1] == bits
+        used = (offset + bits + 63) // 64
+        tail = rng.integers(0, 2 ** 63, size=40, dtype=np.int64).view(np.uint64)
+        tail[5:15] = 0  # a run of zero words, as in a zeroed buffer
+        buf = np.concatenate([words[:used], tail])
+        if offset + bits < used * 64:  # the rest of the last word is the stream's padding
+            buf[used - 1] &= np.uint64((1 << ((offset + bits) & 63)) - 1) if (offset + bits) & 63 else ~np.uint64(0)
+        for L in (128, 512, 4096):
+            out = np.zeros(len(bb) + 1, dtype=np.uint64)
+            rounds = f(t, nn, ss, p[0], p[1], p[2], p[3], buf.ctypes.data_as(u64p), buf.size, offset, L,
+                       out.ctypes.data_as(u64p))
+            assert rounds >= 1
+            assert np.array_equal(out, truth), f"L={L} offset={offset}: first difference at block " \
+                f"{int(np.argmax(out != truth))}"
+        # fewer readable bits than the blocks need: refused
+        if bits > 4096:
+            out = np.zeros(len(bb) + 1, dtype=np.uint64)
+            short = buf[: (offset + bits // 2) // 64]
+            assert f(t, nn, ss, p[0], p[1], p[2], p[3], short.ctypes.data_as(u64p), short.size, offset, 512,
+                     out.ctypes.data_as(u64p)) == -1

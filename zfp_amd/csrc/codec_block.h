@@ -649,4 +649,152 @@ __device__ __forceinline__ u32 decode_block_rev(R& r, S* data, const Geom& g, co
   return c;
 }
 
+// ---------------------------------------------------------------------------
+// Parallel recovery of the block starts of a variable-length stream that
+// arrives without an index.  Following block lengths from a position p
+// gives a chain p -> p + len(p) -> ...; the true chain starts at 0.  The
+// stream's bits [0, span) are cut into P segments of L bits, one lane each:
+//
+//   phase 0  lane i follows the chain from bit i*L (lane 0 from 0) to its
+//            exit, the first chain position at or past the segment's end,
+//            counting the positions before it;
+//   phase 1  (rounds) lane i moves its start to its predecessor's exit of
+//            the round before, t.  If its chain passes through t the two
+//            chains coincide from there and its exit stands (the positions
+//            before t leave its count); otherwise it follows the chain again
+//            from t.  A round in which no exit changes ends the rounds.
+//   phase 2  each lane writes its chain positions at their block numbers
+//            (the exclusive scan of the counts).
+//
+// At the end of the rounds start[0] = 0 and start[i] = exit[i-1], so by
+// induction every start, count and offset is the true chain's: the result
+// is exact, and only the number of rounds depends on how soon chains from
+// arbitrary bits meet the true one (in zfp streams typically within a few
+// hundred blocks).  A float block whose header bit is 0 is one bit long
+// when minbits <= 1, so a run of zero bits is a run of one-bit blocks and
+// is crossed in one step.
+// ---------------------------------------------------------------------------
+struct ChainArgs {
+  Geom g;
+  Params z;
+  const u64* in;
+  u64 nwords, offset, span, L;
+  u64* start;
+  const u64* exit_prev;
+  u64* exit_new;
+  u64* cnt;
+  const u64* base;
+  u64* boffs;
+  u32* changed;
+  u32 P, nblocks;
+};
+
+template <typename S, int D, bool REV>
+__device__ __forceinline__ void chain_lane(const ChainArgs& a, u32 i, u32 phase)
+{
+  const u64 bound = i + 1 == a.P ? a.span : (u64)(i + 1) * a.L;
+  const bool runs = Traits<S>::FLOAT && !REV && a.z.minbits <= 1;
+  u64 pos, limit, t = 0;
+  bool check = false; // phase 1: following the old chain up to t
+  if (phase == 0) {
+    pos = (u64)i * a.L;
+    limit = bound;
+  }
+  else if (phase == 1) {
+    if (i == 0) {
+      a.exit_new[0] = a.exit_prev[0];
+      return;
+    }
+    t = a.exit_prev[i - 1];
+    const u64 s = a.start[i];
+    if (t >= bound) {
+      // the predecessor's chain steps over the whole segment
+      a.start[i] = t;
+      a.cnt[i] = 0;
+      a.exit_new[i] = t;
+      if (t != a.exit_prev[i])
+        *a.changed = 1u;
+      return;
+    }
+    if (s == t) {
+      a.exit_new[i] = a.exit_prev[i];
+      return;
+    }
+    // s < t: does the chain from s pass through t?  s > t: start again at t
+    check = s < t;
+    pos = check ? s : t;
+    limit = check ? t : bound;
+  }
+  else {
+    pos = a.start[i];
+    limit = bound;
+  }
+  u64 k = phase == 2 ? a.base[i] : 0;
+  u32 bc[4] = {0, 0, 0, 0};
+#pragma unroll 1
+  for (int pass = 0; pass < 2; pass++) {
+    u64 steps = 0;
+#pragma unroll 1
+    while (pos < limit) {
+      BitReader r;
+      r.begin(a.in, a.nwords, a.offset + pos);
+      u64 len = 0, n = 1;
+      if (runs) {
+        const u64 v = r.peek64();
+        if (!(v & 1)) {
+          n = v ? ctz64(v) : 64;
+          n = n < limit - pos ? n : limit - pos;
+          len = n;
+        }
+      }
+      if (!len) {
+        if constexpr (REV)
+          len = decode_block_rev<S, D, false>(r, (S*)nullptr, a.g, bc, a.z);
+        else
+          len = decode_block<S, D, false>(r, (S*)nullptr, a.g, bc, a.z);
+        len = len ? len : 1; // blocks are at least one bit when maxbits >= 1
+      }
+      if (phase == 2)
+        for (u64 j = 0; j < n && k + j <= a.nblocks; j++)
+          a.boffs[k + j] = n > 1 ? pos + j : pos;
+      pos += len;
+      steps += n;
+      k += n;
+    }
+    if (phase == 0) {
+      a.start[i] = (u64)i * a.L;
+      a.exit_new[i] = pos;
+      a.cnt[i] = steps;
+      return;
+    }
+    if (phase == 2) {
+      // the first position past the segment is the next segment's first
+      // start, or the end of the last block
+      if (k <= a.nblocks)
+        a.boffs[k] = pos;
+      return;
+    }
+    if (check && pos == t) {
+      // the chain from the old start passes through t
+      a.start[i] = t;
+      a.cnt[i] -= steps;
+      a.exit_new[i] = a.exit_prev[i];
+      return;
+    }
+    if (!check) {
+      // followed again from t
+      a.start[i] = t;
+      a.cnt[i] = steps;
+      a.exit_new[i] = pos;
+      if (pos != a.exit_prev[i])
+        *a.changed = 1u;
+      return;
+    }
+    check = false;
+    pos = t;
+    limit = bound;
+  }
+}
+
+
 } // namespace zfphip

@@ -15,8 +15,10 @@
 //                       blocks that k_decode_list then decodes in full
 //   k_encode_waves<S,D> variable-length write pass whose waves assemble
 //                       their words in LDS (lanes in stream order)
-//   k_parse<S,D>        one-lane sequential parse of block lengths for
-//                       variable-rate streams that arrive without an index
+//   k_chain<S,D>        block starts of a variable-rate stream that arrives
+//                       without an index, by chains of block lengths
+//                       followed in parallel segments (rounds until exact)
+//   k_parse<S,D>        one-lane sequential parse, the fallback of k_chain
 //   k_prepare           zeroes the words that blocks will OR into
 //   k_scan_*            exclusive prefix sum of block lengths
 #pragma once
@@ -26,6 +28,7 @@
 #include <cstring>
 #include <map>
 #include <tuple>
+#include <vector>
 
 #include "backend.h"
 #include "codec_block.h"
@@ -499,11 +502,115 @@ __global__ void k_parse(Geom g, u32 nblocks, Params z, const u64* __restrict__ i
   boffs[nblocks] = pos;
 }
 
+// one lane per segment: the phases of chain_lane (codec_block.h)
+template <typename S, int D, bool REV>
+__global__ void __launch_bounds__(64) k_chain(ChainArgs a, u32 phase)
+{
+  const u32 i = blockIdx.x * 64 + threadIdx.x;
+  if (i < a.P)
+    chain_lane<S, D, REV>(a, i, phase);
+}
+
 static inline unsigned grid_for(u64 n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // reversible mode (zfp_stream_set_reversible, src/zfp.c:750-757) is marked by
 // minexp below ZFP_MIN_EXP; its blocks have variable length
 static inline bool is_reversible(const Params& z) { return z.minexp < -1074; }
+
+// Block offsets (nblocks + 1, relative to `offset`) of a variable-length
+// stream without an index, by the chain rounds of k_chain over the stream's
+// readable bits.  Returns CHAIN_FALLBACK (and writes nothing) when those
+// bits hold fewer than nblocks blocks or the rounds do not settle within
+// ROUNDS_MAX; the caller then parses sequentially.  Waits for `st`.
+constexpr int CHAIN_FALLBACK = 101;
+
+template <typename S, int D>
+int chain_parse(const Desc& d, const Params& z, const u64* in, u64 nwords, u64 offset, u64* boffs, hipStream_t st)
+{
+  constexpr u32 ROUNDS_MAX = 256;
+  const u64 span = nwords * 64 > offset ? nwords * 64 - offset : 0;
+  if (!span || z.maxbits == 0 || d.nblocks == 0)
+    return CHAIN_FALLBACK;
+  // segments of at least 32 Kibit (chains from arbitrary bits mostly meet
+  // the true one within that), at most 65536 of them
+  u64 L = std::max<u64>(32768, (span + 65535) / 65536);
+  L = (L + 63) & ~63ull;
+  const u32 P = (u32)((span + L - 1) / L);
+  void* tp;
+  int rc = g_scan_tmp.get(((u64)5 * P + 1) * sizeof(u64), &tp);
+  if (rc)
+    return rc;
+  u64* start = (u64*)tp;
+  u64* ex[2] = {start + P, start + 2 * (u64)P};
+  u64* cnt = start + 3 * (u64)P;
+  u64* base = start + 4 * (u64)P;
+  u32* changed = (u32*)(start + 5 * (u64)P);
+  ChainArgs a;
+  a.g = d.g;
+  a.z = z;
+  a.in = in;
+  a.nwords = nwords;
+  a.offset = offset;
+  a.span = span;
+  a.L = L;
+  a.start = start;
+  a.cnt = cnt;
+  a.base = base;
+  a.boffs = boffs;
+  a.changed = changed;
+  a.P = P;
+  a.nblocks = d.nblocks;
+  const bool rev = is_reversible(z);
+  auto launch = [&](u32 phase) -> int {
+    if (rev)
+      hipLaunchKernelGGL((k_chain<S, D, true>), dim3(grid_for(P, 64)), dim3(64), 0, st, a, phase);
+    else
+      hipLaunchKernelGGL((k_chain<S, D, false>), dim3(grid_for(P, 64)), dim3(64), 0, st, a, phase);
+    HIPCHK(hipGetLastError());
+    return 0;
+  };
+  a.exit_prev = ex[1];
+  a.exit_new = ex[0];
+  if ((rc = launch(0)))
+    return rc;
+  int cur = 0; // ex[cur] holds the latest exits
+  u32 rounds = 0;
+  for (;; rounds++) {
+    if (rounds == ROUNDS_MAX)
+      return CHAIN_FALLBACK;
+    HIPCHK(hipMemsetAsync(changed, 0, sizeof(u32), st));
+    a.exit_prev = ex[cur];
+    a.exit_new = ex[cur ^ 1];
+    if ((rc = launch(1)))
+      return rc;
+    cur ^= 1;
+    u32 h = 0;
+    HIPCHK(hipMemcpyAsync(&h, changed, sizeof(u32), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (!h)
+      break;
+  }
+  // block numbers of the segments' first starts
+  std::vector<u64> c(P);
+  HIPCHK(hipMemcpyAsync(c.data(), cnt, (u64)P * sizeof(u64), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  u64 total = 0;
+  for (u32 i = 0; i < P; i++) {
+    const u64 v = c[i];
+    c[i] = total;
+    total += v;
+  }
+  if (total < d.nblocks)
+    return CHAIN_FALLBACK;
+  HIPCHK(hipMemcpyAsync(base, c.data(), (u64)P * sizeof(u64), hipMemcpyHostToDevice, st));
+  a.exit_prev = ex[cur];
+  a.exit_new = ex[cur ^ 1];
+  if ((rc = launch(2)))
+    return rc;
+  // the host vector must outlive the copy
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
 
 template <typename S, int D, bool REV>
 int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits_out,
@@ -715,13 +822,18 @@ int launch_decode(const Desc& d, void* data, const Params& z, const u64* in, u64
     int rc = g_lens.get(((u64)d.nblocks + 1) * sizeof(u64), &lp);
     if (rc)
       return rc;
-    if (rev)
-      hipLaunchKernelGGL((k_parse<S, D, true>), dim3(1), dim3(64), 0, st, d.g, d.nblocks, z, in, nwords,
-                         offset, (u64*)lp);
-    else
-      hipLaunchKernelGGL((k_parse<S, D, false>), dim3(1), dim3(64), 0, st, d.g, d.nblocks, z, in, nwords,
-                         offset, (u64*)lp);
-    HIPCHK(hipGetLastError());
+    rc = chain_parse<S, D>(d, z, in, nwords, offset, (u64*)lp, st);
+    if (rc == CHAIN_FALLBACK) {
+      if (rev)
+        hipLaunchKernelGGL((k_parse<S, D, true>), dim3(1), dim3(64), 0, st, d.g, d.nblocks, z, in, nwords,
+                           offset, (u64*)lp);
+      else
+        hipLaunchKernelGGL((k_parse<S, D, false>), dim3(1), dim3(64), 0, st, d.g, d.nblocks, z, in, nwords,
+                           offset, (u64*)lp);
+      HIPCHK(hipGetLastError());
+    }
+    else if (rc)
+      return rc;
     boffs = (const u64*)lp;
   }
   // fixed rate: stage blocks through LDS when a workgroup's slots fit in
