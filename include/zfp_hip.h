@@ -137,6 +137,25 @@ int zfp_hip_decode_blocks(int type, unsigned dims, size_t count, void* d_blocks,
                           uint minbits, uint maxbits, uint maxprec, int minexp, const uint64_t* d_words,
                           uint64_t nwords, uint64_t offset, uint64_t* bits_out, void* hip_stream);
 
+/*
+ * The same block codec for compressed arrays held in host memory, the
+ * batched replacement of zfp::codec::zfp_base::encode_block/decode_block
+ * (include/zfp/codec/zfpcodec.hpp:146-190) used by the codec classes of
+ * include/zfp/hipcodec.hpp.  blocks: `count` whole blocks of 4^dims values
+ * in host memory, one after another; slots: host array of count slot
+ * numbers; words/nwords: the array's host stream.  Fixed rate with maxbits
+ * and offset multiples of 64 (the rates compressed arrays use): block k is
+ * written to / read from bits offset + slots[k] * maxbits, and no other
+ * word of the stream is touched.  A slot past nwords fails the call
+ * (ZFP_HIP_EINVAL) before anything is written.  type: zfp_type (1-4).
+ */
+int zfp_hip_encode_blocks_host(int type, unsigned dims, size_t count, const void* blocks, const uint64_t* slots,
+                               uint minbits, uint maxbits, uint maxprec, int minexp, uint64_t* words,
+                               uint64_t nwords, uint64_t offset);
+int zfp_hip_decode_blocks_host(int type, unsigned dims, size_t count, void* blocks, const uint64_t* slots,
+                               uint minbits, uint maxbits, uint maxprec, int minexp, const uint64_t* words,
+                               uint64_t nwords, uint64_t offset);
+
 /* Number of per-block bit offsets recorded for the stream whose buffer
    begins at `stream_begin` (0 if none); and export/import of that index as
    64-bit absolute bit offsets, one per block plus the end offset, so that a

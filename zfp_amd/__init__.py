@@ -166,6 +166,14 @@ _HIP = {
                                              ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_int,
                                              ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                              _P(ctypes.c_uint64), ctypes.c_void_p]),
+    "zfp_hip_encode_blocks_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint, ctypes.c_size_t, ctypes.c_void_p,
+                                                  _P(ctypes.c_uint64), ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                                  ctypes.c_int, _P(ctypes.c_uint64), ctypes.c_uint64,
+                                                  ctypes.c_uint64]),
+    "zfp_hip_decode_blocks_host": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint, ctypes.c_size_t, ctypes.c_void_p,
+                                                  _P(ctypes.c_uint64), ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                                  ctypes.c_int, _P(ctypes.c_uint64), ctypes.c_uint64,
+                                                  ctypes.c_uint64]),
     "zfp_hip_index_size": (ctypes.c_size_t, [ctypes.c_void_p]),
     "zfp_hip_index_export_hybrid4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                                     _P(ctypes.c_uint64), _P(ctypes.c_uint64)]),

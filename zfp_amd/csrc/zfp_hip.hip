@@ -604,6 +604,86 @@ extern "C" int zfp_hip_decode_blocks(int type, unsigned dims, size_t count, void
   return ZFP_HIP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The block codec for arrays held in host memory (zfp::codec::hip3 and the
+// other codecs of include/zfp/hipcodec.hpp): blocks and the stream stay on
+// the host; the blocks go to the device as one contiguous run, are coded
+// there as a compact fixed-rate stream (block k at k * maxbits), and each
+// block's words are moved between that stream and its slot of the caller's.
+// ---------------------------------------------------------------------------
+namespace {
+
+std::mutex g_host_blocks_mutex;
+Scratch g_host_blocks, g_host_words;
+
+int host_blocks_check(int type, unsigned dims, size_t count, const void* blocks, const uint64_t* slots,
+                      uint minbits, uint maxbits, int minexp, const void* words, uint64_t nwords, uint64_t offset)
+{
+  if (type < 1 || type > 4 || dims < 1 || dims > 4 || !count || count > (1ull << 29) || !blocks || !slots || !words)
+    return record_error(ZFP_HIP_EINVAL, "invalid block batch");
+  if (!fixed_rate(Params{minbits, maxbits, 64, minexp}) || !maxbits || (maxbits & 63) || (offset & 63))
+    return record_error(ZFP_HIP_EINVAL, "host block batches need fixed rate with word-aligned blocks");
+  const u64 nw = maxbits / 64, w0 = offset / 64;
+  for (size_t k = 0; k < count; k++)
+    if (slots[k] > (nwords - std::min<u64>(nwords, w0)) / nw || w0 + (slots[k] + 1) * nw > nwords)
+      return record_error(ZFP_HIP_EINVAL, "a slot lies past the end of the stream");
+  return ZFP_HIP_OK;
+}
+
+} // namespace
+
+extern "C" int zfp_hip_encode_blocks_host(int type, unsigned dims, size_t count, const void* blocks,
+                                          const uint64_t* slots, uint minbits, uint maxbits, uint maxprec,
+                                          int minexp, uint64_t* words, uint64_t nwords, uint64_t offset)
+{
+  int rc = host_blocks_check(type, dims, count, blocks, slots, minbits, maxbits, minexp, words, nwords, offset);
+  if (rc)
+    return rc;
+  std::lock_guard<std::mutex> lock(g_host_blocks_mutex);
+  const size_t esize = type == 1 || type == 3 ? 4 : 8;
+  const size_t bbytes = count * ((size_t)1 << (2 * dims)) * esize;
+  const u64 nw = maxbits / 64;
+  void *db, *dw;
+  if ((rc = g_host_blocks.get(bbytes, &db)) || (rc = g_host_words.get(count * nw * 8, &dw)))
+    return rc;
+  HIPCHK(hipMemcpy(db, blocks, bbytes, hipMemcpyHostToDevice));
+  u64 bits = 0;
+  if ((rc = zfp_hip_encode_blocks(type, dims, count, db, nullptr, minbits, maxbits, maxprec, minexp, (u64*)dw,
+                                  count * nw, 0, &bits, nullptr)))
+    return rc;
+  std::vector<u64> compact(count * nw);
+  HIPCHK(hipMemcpy(compact.data(), dw, count * nw * 8, hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < count; k++)
+    std::memcpy(words + offset / 64 + slots[k] * nw, compact.data() + k * nw, nw * 8);
+  return ZFP_HIP_OK;
+}
+
+extern "C" int zfp_hip_decode_blocks_host(int type, unsigned dims, size_t count, void* blocks, const uint64_t* slots,
+                                          uint minbits, uint maxbits, uint maxprec, int minexp,
+                                          const uint64_t* words, uint64_t nwords, uint64_t offset)
+{
+  int rc = host_blocks_check(type, dims, count, blocks, slots, minbits, maxbits, minexp, words, nwords, offset);
+  if (rc)
+    return rc;
+  std::lock_guard<std::mutex> lock(g_host_blocks_mutex);
+  const size_t esize = type == 1 || type == 3 ? 4 : 8;
+  const size_t bbytes = count * ((size_t)1 << (2 * dims)) * esize;
+  const u64 nw = maxbits / 64;
+  void *db, *dw;
+  if ((rc = g_host_blocks.get(bbytes, &db)) || (rc = g_host_words.get(count * nw * 8, &dw)))
+    return rc;
+  std::vector<u64> compact(count * nw);
+  for (size_t k = 0; k < count; k++)
+    std::memcpy(compact.data() + k * nw, words + offset / 64 + slots[k] * nw, nw * 8);
+  HIPCHK(hipMemcpy(dw, compact.data(), count * nw * 8, hipMemcpyHostToDevice));
+  u64 bits = 0;
+  if ((rc = zfp_hip_decode_blocks(type, dims, count, db, nullptr, minbits, maxbits, maxprec, minexp, (u64*)dw,
+                                  count * nw, 0, &bits, nullptr)))
+    return rc;
+  HIPCHK(hipMemcpy(blocks, db, bbytes, hipMemcpyDeviceToHost));
+  return ZFP_HIP_OK;
+}
+
 extern "C" size_t zfp_hip_compress(zfp_stream* zs, const zfp_field* field)
 {
   if (!zs || !zs->stream || !field)
