@@ -10,10 +10,11 @@ sinusoids (wavenumbers 1-6, amplitudes U(0.1, 1)) plus N(0, 1e-3) noise, with
 One step = compress + decompress at each of the three rates for both layouts
 (12 codec passes over 1 GiB).  `value` is the aggregate uncompressed
 throughput of those passes: 12 * 1 GiB / step time.  The roofline entry is
-for the dominant kernel (compression at rate 16, contiguous): algorithmic
-bytes N*8 + C per launch over its mean duration, timed with HIP events on
-the stream it is launched on; its `traffic` is the HBM bytes per launch that
-the committed rocprofv3 counter summary (profiles/) measured.  The CPU
+for the pass with the largest mean time per step: algorithmic bytes N*8 + C
+per launch over its mean duration, timed with HIP events on the stream it is
+launched on; its `traffic` is the HBM bytes per launch that the committed
+rocprofv3 counter summary (profiles/) measured for that pass.  `passes`
+gives every pass's time and HBM fraction.  The CPU
 baseline is the reference's serial zfp_compress/zfp_decompress (and its
 OpenMP compression) on a 256^3 slab (128 MiB) of the same field, timed on
 this machine's host cores: one warm-up repetition, then five or more within
@@ -22,7 +23,8 @@ a budget of about 20 s of CPU time.
 Beside the headline line's own workload, the same JSON line reports:
   batched        1024 mixed arrays in one persistent launch per pass (graph
                  replay) against one zfp_hip_encode/decode call per array;
-  serving        the request loop at three Poisson offered loads (p50/p99);
+  serving        the request loop at three Poisson offered loads (p50/p99,
+                 overall and per request kind);
   variable_rate  precision 16/28 and accuracy 1e-3/1e-7 on a 2D float32
                  8191x8193 and a 4D float64 96x96x96x24 array.
 
@@ -552,9 +554,18 @@ def serve_bench(args):
         lam = L * capacity / mean_req
         arr = np.cumsum(rng.exponential(1.0 / lam, size=R))
         res, t_run, st = run(arr)
-        lat = sorted(x[1] for x in res)
-        loads[str(L)] = {"offered_req_per_s": round(lam, 1), "p50_ms": round(lat[len(lat) // 2], 3),
-                         "p99_ms": round(lat[min(R - 1, int(0.99 * (R - 1) + 0.5))], 3),
+
+        def pct(lat):
+            lat = sorted(lat)
+            n = len(lat)
+            return (round(lat[n // 2], 3), round(lat[min(n - 1, int(0.99 * (n - 1) + 0.5))], 3)) if n else (None, None)
+
+        p50, p99 = pct([x[1] for x in res])
+        c50, c99 = pct([x[1] for x, (_, kind) in zip(res, picks) if kind == "c"])
+        d50, d99 = pct([x[1] for x, (_, kind) in zip(res, picks) if kind == "d"])
+        loads[str(L)] = {"offered_req_per_s": round(lam, 1), "p50_ms": p50, "p99_ms": p99,
+                         "compress_p50_ms": c50, "compress_p99_ms": c99,
+                         "decompress_p50_ms": d50, "decompress_p99_ms": d99,
                          "GBps": round(ubytes / t_run / 1e9, 1), "batches": st["batches"],
                          "max_batch": st["max_batch"], "max_inflight": st["max_inflight"]}
         # every result equals the job run alone

@@ -190,3 +190,51 @@ def test_inputs_produced_on_a_side_stream(reqs):
     assert bits == r.bits
     got = w.cpu().numpy().view(np.uint64)[: (bits + 63) // 64]
     assert np.array_equal(got, r.words[: len(got)])
+
+
+def test_requests_on_one_buffer_keep_their_order(reqs):
+    """compress(A -> W) then decompress(W -> B), back to back: B is the
+    oracle's reconstruction of A; decompress(W -> B2) then compress(C -> W):
+    B2 is still A's reconstruction and W ends as C's stream; and two
+    compressions into one buffer leave the second's stream."""
+    torch = _torch()
+    a = next(r for r in reqs if not r.var and r.ztype == T_DOUBLE)
+    w = torch.zeros(max(a.nwords, 8), dtype=torch.int64, device="cuda:0")
+    b = torch.zeros_like(a.field)
+    b2 = torch.zeros_like(a.field)
+    comp = Z.make_job(a.ztype, a.n, None, a.field.data_ptr(), a.params, w.data_ptr(), 0, a.nwords, None)
+    dec = Z.make_job(a.ztype, a.n, None, b.data_ptr(), a.params, w.data_ptr(), 0, a.nwords, None)
+    dec2 = Z.make_job(a.ztype, a.n, None, b2.data_ptr(), a.params, w.data_ptr(), 0, a.nwords, None)
+    # a second field of the same shape and mode: A's field perturbed
+    other = (a.field * 0.5 + 0.25).contiguous()
+    host_other = other.cpu().numpy()
+    ow, obits = oracle_compress(host_other, 0, a.n, None, a.ztype, a.params)
+    comp2 = Z.make_job(a.ztype, a.n, None, other.data_ptr(), a.params, w.data_ptr(), 0, a.nwords, None)
+    sv = Z.Server(64 << 30)
+    for rep in range(20):
+        w.zero_()
+        b.zero_()
+        b2.zero_()
+        torch.cuda.synchronize()
+        i1 = sv.submit(comp, Z.ZFP_HIP_BATCH_COMPRESS, a.nbytes)
+        i2 = sv.submit(dec, Z.ZFP_HIP_BATCH_DECOMPRESS, a.nbytes)      # reads what i1 writes
+        i3 = sv.submit(dec2, Z.ZFP_HIP_BATCH_DECOMPRESS, a.nbytes)     # reads what i1 writes
+        i4 = sv.submit(comp2, Z.ZFP_HIP_BATCH_COMPRESS, a.nbytes)      # overwrites what i2, i3 read
+        assert sv.wait(i1)[0] == a.bits
+        assert sv.wait(i2)[0] == a.bits
+        assert sv.wait(i3)[0] == a.bits
+        assert sv.wait(i4)[0] == obits
+        assert np.array_equal(b.cpu().numpy().view(np.uint8), a.rec.view(np.uint8)), f"repetition {rep}"
+        assert np.array_equal(b2.cpu().numpy().view(np.uint8), a.rec.view(np.uint8)), f"repetition {rep}"
+        got = w.cpu().numpy().view(np.uint64)[: (obits + 63) // 64]
+        assert np.array_equal(got, ow[: len(got)]), f"repetition {rep}"
+        # two compressions into one buffer: the later one's stream remains
+        w.zero_()
+        torch.cuda.synchronize()
+        j1 = sv.submit(comp2, Z.ZFP_HIP_BATCH_COMPRESS, a.nbytes)
+        j2 = sv.submit(comp, Z.ZFP_HIP_BATCH_COMPRESS, a.nbytes)
+        sv.wait(j1)
+        sv.wait(j2)
+        got = w.cpu().numpy().view(np.uint64)[: (a.bits + 63) // 64]
+        assert np.array_equal(got, a.words[: len(got)]), f"repetition {rep}"
+    sv.close()

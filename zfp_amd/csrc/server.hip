@@ -1,7 +1,11 @@
 // server.hip -- the serving loop of include/zfp_hip.h: requests are admitted
 // in arrival order while their declared footprint fits the HBM budget, every
 // admitted request that is not yet running joins the next batch, and up to
-// `slots` batches run at once, each on its own HIP stream.  A batch runs the
+// `slots` batches run at once, each on its own HIP stream.  Requests that
+// touch the same memory keep their submission order: one that reads or
+// writes what an earlier one writes (or writes what it reads) waits for it,
+// except a decompression behind a compression in the same batch, which a
+// batch runs in that order anyway.  A batch runs the
 // persistent kernels of the batched codec (batch_kernels.h) with a launch
 // shape fixed at creation.  A scheduler thread owns the streams; clients
 // only queue requests and wait for them.
@@ -20,9 +24,18 @@
 #include "batch_kernels.h"
 
 namespace zfphip {
+
+struct Server;
+
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// byte range [lo, hi) of device memory
+struct Span {
+  uintptr_t lo = 0, hi = 0;
+  bool meets(const Span& o) const { return lo < o.hi && o.lo < hi; }
+};
 
 struct Request {
   u64 id = 0;
@@ -33,7 +46,80 @@ struct Request {
   Clock::time_point t_submit, t_done;
   u64 bits = 0;
   int status = -1;            // -1 while queued or running
+  Span rd[2], wr[2];          // memory the request reads and writes
 };
+
+// The field's bytes: element 0 at data, any strides (batch_kernels.h BJob)
+Span field_span(const BJob& J)
+{
+  const u32 type = J.kind / 3 + 1;
+  const i64 es = type == 1 || type == 3 ? 4 : 8;
+  i64 lo = 0, hi = 0;
+  for (int d = 0; d < 4; d++) {
+    const i64 e = (i64)(J.g.n[d] ? J.g.n[d] - 1 : 0) * J.g.s[d];
+    (e < 0 ? lo : hi) += e;
+  }
+  const uintptr_t b = (uintptr_t)J.data;
+  return Span{b + (uintptr_t)(lo * es), b + (uintptr_t)((hi + 1) * es)};
+}
+
+// What a request reads and writes: compression reads the field and writes
+// the stream words its blocks can reach (and the index); decompression
+// reads the stream's readable words (and the index) and writes the field.
+void request_spans(Request& r)
+{
+  const BJob& J = r.job;
+  const u32 dims = J.kind % 3 + 1, type = J.kind / 3 + 1;
+  const u64 size = 1ull << (2 * dims), P = type == 1 || type == 3 ? 32 : 64;
+  u64 words;
+  if (r.dir == ZFP_HIP_BATCH_DECOMPRESS)
+    words = J.nwords;
+  else {
+    // longest block: header, group tests and maxprec planes, within the
+    // budget (zfp_stream_maximum_size, src/zfp.c:711-742)
+    const u64 most = 12 + size - 1 + size * std::min<u64>(J.z.maxprec, P);
+    const u64 per = std::max<u64>(J.z.minbits, std::min<u64>(J.z.maxbits, most));
+    words = (J.offset + (u64)J.nblocks * per + 63) / 64;
+  }
+  const uintptr_t w = (uintptr_t)J.words;
+  const Span stream{w, w + words * 8};
+  const Span index = J.index ? Span{(uintptr_t)J.index, (uintptr_t)(J.index + J.nblocks + 1)} : Span{};
+  const Span field = field_span(J);
+  if (r.dir == ZFP_HIP_BATCH_DECOMPRESS) {
+    r.rd[0] = stream;
+    r.rd[1] = index;
+    r.wr[0] = field;
+  }
+  else {
+    r.rd[0] = field;
+    r.wr[0] = stream;
+    r.wr[1] = index;
+  }
+}
+
+bool meets(const Span* a, const Span* b)
+{
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      if (a[i].hi > a[i].lo && b[j].hi > b[j].lo && a[i].meets(b[j]))
+        return true;
+  return false;
+}
+
+// whether request b, submitted after a, has to wait for a: one writes what
+// the other reads or writes
+bool depends(const Request& a, const Request& b)
+{
+  return meets(a.wr, b.rd) || meets(a.rd, b.wr) || meets(a.wr, b.wr);
+}
+
+// whether b may still run in the same batch as a (a earlier): a batch runs
+// every compression before any decompression, so a decompression that
+// depends only on earlier compressions keeps their order
+bool same_batch_ok(const Request& a, const Request& b)
+{
+  return !depends(a, b) || (a.dir == ZFP_HIP_BATCH_COMPRESS && b.dir == ZFP_HIP_BATCH_DECOMPRESS);
+}
 
 // grow-only device and pinned host buffers
 template <typename T>
@@ -100,6 +186,8 @@ struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
+  bool signalled = false;     // set by the stream's host function at the batch's end
+  Server* server = nullptr;
   std::vector<Request*> reqs;
   PinBuf<BJob> h_jobs;
   DevBuf<BJob> d_jobs;
@@ -224,7 +312,21 @@ struct Server {
       return rc;
     HIPCHK(hipMemcpyAsync(s.h_bits.p, s.d_bits.p, n * sizeof(u64), hipMemcpyDeviceToHost, s.st));
     HIPCHK(hipEventRecord(s.done, s.st));
+    HIPCHK(hipLaunchHostFunc(s.st, &Server::batch_done, &s));
     return ZFP_HIP_OK;
+  }
+
+  // runs on a runtime thread once slot s's batch (and the copy of its bit
+  // counts) is complete: wakes the scheduler
+  static void batch_done(void* arg)
+  {
+    Slot* s = static_cast<Slot*>(arg);
+    Server* sv = s->server;
+    {
+      std::lock_guard<std::mutex> lk(sv->m);
+      s->signalled = true;
+    }
+    sv->cv_work.notify_all();
   }
 
   // completes the requests of slot s (lock held)
@@ -241,34 +343,66 @@ struct Server {
     }
     s.reqs.clear();
     s.busy = false;
+    s.signalled = false;
     cv_done.notify_all();
+  }
+
+  // whether r may join `batch` now: no conflict with a request in flight,
+  // with an earlier request left waiting, or (but for compress-then-
+  // decompress) with one already in the batch
+  bool admissible(const Request& r, const std::vector<Request*>& batch, const std::vector<Request*>& held) const
+  {
+    for (const Slot& t : slots)
+      if (t.busy)
+        for (const Request* a : t.reqs)
+          if (depends(*a, r))
+            return false;
+    for (const Request* a : held)
+      if (depends(*a, r))
+        return false;
+    for (const Request* a : batch)
+      if (!same_batch_ok(*a, r))
+        return false;
+    return true;
   }
 
   void loop()
   {
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
-      // completions
+      // completions: signalled by the streams' host functions; the event
+      // query also catches a batch whose stream failed
       for (Slot& s : slots) {
         if (!s.busy)
           continue;
         const hipError_t q = hipEventQuery(s.done);
-        if (q == hipSuccess)
+        if (q == hipSuccess && s.signalled)
           finish(s, ZFP_HIP_OK);
-        else if (q != hipErrorNotReady) {
+        else if (q != hipSuccess && q != hipErrorNotReady) {
           (void)hipGetLastError();
           finish(s, ZFP_HIP_ERUNTIME);
         }
       }
-      // admission and launches: the queue head waits until it fits
+      // admission and launches, in arrival order: a request that does not
+      // fit the budget stops the admission (it waits at the head); one that
+      // touches memory of a request in flight or of an earlier waiting one
+      // waits, and later requests pass it
       for (Slot& s : slots) {
         if (s.busy || pending.empty())
           continue;
-        std::vector<Request*> batch;
-        while (!pending.empty() && batch.size() < max_jobs && used + pending.front()->bytes <= budget) {
-          batch.push_back(pending.front());
-          used += pending.front()->bytes;
-          pending.pop_front();
+        std::vector<Request*> batch, held;
+        for (auto it = pending.begin(); it != pending.end() && batch.size() < max_jobs;) {
+          Request* r = *it;
+          if (used + r->bytes > budget)
+            break;
+          if (!admissible(*r, batch, held)) {
+            held.push_back(r);
+            ++it;
+            continue;
+          }
+          batch.push_back(r);
+          used += r->bytes;
+          it = pending.erase(it);
         }
         if (batch.empty())
           break;
@@ -294,8 +428,10 @@ struct Server {
         busy = busy || s.busy;
       if (stop && pending.empty() && !busy)
         break;
+      // woken by submissions and batch ends; the timeout is only a backstop
+      // for a stream that fails before its host function runs
       if (busy)
-        cv_work.wait_for(lk, std::chrono::microseconds(20));
+        cv_work.wait_for(lk, std::chrono::milliseconds(5));
       else
         cv_work.wait(lk, [this] { return stop || !pending.empty(); });
     }
@@ -363,6 +499,7 @@ extern "C" zfp_hip_server* zfp_hip_server_create(uint64_t hbm_budget, unsigned m
   (void)hipGetLastError();
   sv->slots.resize(slots ? slots : 2);
   for (Slot& s : sv->slots) {
+    s.server = sv;
     if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
       (void)hipGetLastError();
@@ -404,6 +541,7 @@ extern "C" int zfp_hip_server_submit(zfp_hip_server* server, const zfp_hip_job* 
   }
   r->dir = direction;
   r->bytes = bytes;
+  request_spans(*r);
   r->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(sv->m);
