@@ -86,10 +86,11 @@ struct Scratch {
   }
 };
 
-// scratch of the variable-rate passes (lengths + offsets, scan tiles) and
-// the index-check flag of variable-rate decoding; all entry points run
-// under one library lock, so one set suffices
-extern Scratch g_lens, g_scan_tmp, g_flag;
+// scratch of the variable-rate passes (lengths + offsets, scan tiles), the
+// index-check flag of variable-rate decoding, and the staged bits of
+// variable-rate coding (encode_once) and the chain rounds of index-less
+// decoding; all entry points run under one library lock, so one set suffices
+extern Scratch g_lens, g_scan_tmp, g_flag, g_stage;
 
 // internal status of dispatch_decode (never returned by the C ABI): a
 // block's length disagrees with the block index the caller supplied, so the

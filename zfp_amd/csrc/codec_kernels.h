@@ -336,6 +336,160 @@ k_encode_waves(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* _
   }
 }
 
+// Bits one block can take in the stream's mode: the per-block term of
+// zfp_stream_maximum_size (src/zfp.c:711-742).
+template <typename S, int D, bool REV>
+__host__ __device__ inline u32 block_bits_bound(const Params& z)
+{
+  typedef Traits<S> T;
+  constexpr u32 N = 1u << (2 * D);
+  const u32 hdr = T::FLOAT ? (REV ? 2 + T::EBITS + T::PBITS : 1 + T::EBITS) : (REV ? T::PBITS : 0);
+  const u32 prec = z.maxprec < (u32)T::P ? z.maxprec : (u32)T::P;
+  u64 b = hdr + (N - 1) + (u64)N * prec;
+  b = b < z.maxbits ? b : z.maxbits;
+  b = b > z.minbits ? b : z.minbits;
+  return (u32)b;
+}
+
+// Variable-length coding that codes every block once, for fields whose
+// lanes hold consecutive stream blocks of at most bw words each:
+//
+//   k_encode_stage  each wave codes its 64 blocks into LDS slots,
+//                   concatenates their bits in LDS and stores them, from bit
+//                   0, in its own stride of 64 * bw words of `stage`; it
+//                   writes its bit count to totals[wave] and each block's
+//                   offset within the wave to boffs;
+//   exclusive scan  of the totals (one per wave): each wave's first bit;
+//   k_place         each staged wave's bits are shifted into place (the
+//                   first word keeps the bits before them), and the block
+//                   offsets get their wave's first bit.
+//
+// No wave waits for another and no word is shared between writers.
+template <typename S, int D, bool REV>
+__global__ void __launch_bounds__(256)
+k_encode_stage(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* __restrict__ stage,
+               u64* __restrict__ totals, u64* __restrict__ boffs, u32 bw)
+{
+  extern __shared__ u64 lds[];
+  const u32 lane = threadIdx.x & 63u;
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x, wn = t >> 6;
+  if ((t & ~63u) >= nblocks)
+    return; // the whole wave lies past the field
+  const u32 sstride = (bw + 2) | 1u, awords = 64 * bw + 2;
+  u64* slot_base = lds + (u64)(threadIdx.x >> 6) * (64 * sstride + awords);
+  u64* area = slot_base + 64 * sstride;
+  const bool act = t < nblocks;
+  u64* slot = slot_base + lane * sstride;
+  for (u32 i = 0; i < sstride; i++)
+    slot[i] = 0;
+  u64 len = 0;
+  if (act) {
+    u32 bc[4] = {0, 0, 0, 0};
+    lane_block<D>(t, g, bc); // == t: stream order
+    SlotWriter w;
+    w.begin(slot, bw + 2);
+    len = encode_any<S, D, REV>(w, data, g, bc, z);
+  }
+  // bit offsets of the lanes' blocks within the wave
+  u64 incl = len;
+#pragma unroll
+  for (u32 d = 1; d < 64; d <<= 1) {
+    const u64 v = __shfl_up(incl, d, 64);
+    if (lane >= d)
+      incl += v;
+  }
+  const u64 excl = incl - len;
+  const u64 T = __shfl(incl, 63, 64);
+  const u32 nwa = (u32)((T + 63) >> 6);
+  for (u32 i = lane; i <= nwa; i += 64)
+    area[i] = 0;
+  wave_lds_sync();
+  if (act && len) {
+    // area words fb..lb hold the block; those it fills whole are stored,
+    // the two it shares with its neighbours are OR-ed
+    const u32 sh = (u32)(excl & 63), fb = (u32)(excl >> 6), lb = (u32)((excl + len - 1) >> 6);
+    const u32 lw = (u32)((len + 63) >> 6);
+    u64 prev = 0;
+    for (u32 k = fb, i = 0; k <= lb; k++, i++) {
+      u64 cur = i < lw ? slot[i] : 0ull;
+      if (i + 1 == lw && (len & 63))
+        cur &= lowmask((u32)(len & 63));
+      const u64 v = sh ? (cur << sh) | (prev >> (64 - sh)) : cur;
+      prev = cur;
+      const bool whole = (u64)k * 64 >= excl && (u64)k * 64 + 64 <= excl + len;
+      if (whole)
+        area[k] = v;
+      else if (v)
+        atomicOr((unsigned long long*)(area + k), (unsigned long long)v);
+    }
+  }
+  wave_lds_sync();
+  u64* dst = stage + (u64)wn * 64 * bw;
+  for (u32 i = lane; i < nwa; i += 64)
+    dst[i] = area[i];
+  if (lane == 0)
+    totals[wn] = T;
+  if (act)
+    boffs[t] = excl;
+}
+
+// Bits [pos, pos + n) (n <= 64) of wave w's staged bits, low bit first
+ZH_INLINE u64 staged_bits(const u64* __restrict__ stage, u64 stride, u64 w, u64 pos, u32 n)
+{
+  const u64* q = stage + w * stride + (pos >> 6);
+  const u32 s = (u32)(pos & 63);
+  u64 v = q[0] >> s;
+  if (s && s + n > 64)
+    v |= q[1] << (64 - s);
+  return v & lowmask(n);
+}
+
+// One wave per staged wave w: its bits shifted to their place in `out`.
+// The first word takes the bits before them (the previous wave's last
+// bits, or the caller's below `offset`); the last word, when the next wave
+// shares it, is left to that wave.  Every staged wave but the last holds at
+// least 64 bits (each block takes at least one), so the previous wave's
+// last bits are in its own stride.  (A template, so that every translation
+// unit that includes this header may instantiate it.)
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(256)
+k_place(const u64* __restrict__ stage, u64 stride, const u64* __restrict__ first, u32 nwaves, u32 nblocks,
+        u64* __restrict__ out, u64 offset, u64* __restrict__ boffs)
+{
+  const u32 lane = threadIdx.x & 63u;
+  const u32 w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (w >= nwaves)
+    return;
+  const u64 T = first[w + 1] - first[w];
+  const u64 G = offset + first[w];
+  const u32 s = (u32)(G & 63);
+  const u64 a = G >> 6, nout = (s + T + 63) >> 6;
+  const bool last = w + 1 == nwaves;
+  const u64* src = stage + (u64)w * stride;
+  for (u64 j = lane; j < nout; j += 64) {
+    const u64 lo = j < (T + 63) >> 6 ? src[j] : 0ull;
+    const u64 prev = j ? src[j - 1] : 0ull;
+    u64 v = s ? (lo << s) | (prev >> (64 - s)) : lo;
+    if (j == 0 && s) {
+      if (w) {
+        const u64 Tp = first[w] - first[w - 1];
+        v |= staged_bits(stage, stride, w - 1, Tp - s, s);
+      }
+      else
+        v |= out[a] & lowmask(s);
+    }
+    const bool shared = j + 1 == nout && ((s + T) & 63) && !last;
+    if (!shared)
+      out[a + j] = v;
+  }
+  // block offsets: the wave's first bit added
+  const u32 t = w * 64 + lane;
+  if (t < nblocks)
+    boffs[t] += first[w];
+  if (last && lane == 0)
+    boffs[nblocks] = first[nwaves];
+}
+
 template <typename S, int D, bool REV, bool HI_ONLY, typename R>
 __device__ __forceinline__ void decode_lane(R& r, S* __restrict__ data, const Geom& g, const u32* bc,
                                             const Params& z, u32 t, u32 b, const u64* __restrict__ boffs,
@@ -687,8 +841,45 @@ int launch_encode(const Desc& d, const void* data, const Params& z, u64* out, u6
   }
 }
 
+// Coding every block once (k_encode_stage, a scan of the waves' bit counts,
+// k_place) for a field whose lanes hold consecutive stream blocks, each of at
+// most bw words; block offsets go to offs (nblocks + 1).  Waits for `st`.
+template <typename S, int D, bool REV>
+int encode_once(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* offs, u32 bw,
+                u64* total, hipStream_t st)
+{
+  const u32 nwaves = (u32)((d.nblocks + 63) / 64);
+  const u64 stride = (u64)64 * bw;
+  void* tp;
+  // the waves' bit counts and first bits, then the staged bits
+  int rc = g_stage.get(((u64)2 * (nwaves + 1) + (u64)nwaves * stride) * sizeof(u64), &tp);
+  if (rc)
+    return rc;
+  u64* totals = (u64*)tp;
+  u64* first = totals + nwaves + 1;
+  u64* stage = first + nwaves + 1;
+  const u32 sstride = (bw + 2) | 1u;
+  const size_t per_wave = ((size_t)64 * sstride + 64 * bw + 2) * 8;
+  unsigned wg = 256;
+  while (wg > 64 && (wg / 64) * per_wave > 65536)
+    wg /= 2;
+  HIPCHK(hipMemsetAsync(totals + nwaves, 0, sizeof(u64), st));
+  hipLaunchKernelGGL((k_encode_stage<S, D, REV>), dim3(grid_for(d.nblocks, wg)), dim3(wg), (wg / 64) * per_wave,
+                     st, (const S*)data, d.g, d.nblocks, z, stage, totals, offs, bw);
+  HIPCHK(hipGetLastError());
+  if ((rc = exclusive_scan(totals, first, (u64)nwaves + 1, st)))
+    return rc;
+  hipLaunchKernelGGL(k_place<0>, dim3(grid_for(nwaves, 4)), dim3(256), 0, st, (const u64*)stage, stride,
+                     (const u64*)first, nwaves, d.nblocks, out, offset, offs);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(total, first + nwaves, sizeof(u64), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return ZFP_HIP_OK;
+}
+
 // variable rate (and reversible): length pass, exclusive scan, prepare,
-// write pass
+// write pass -- or, for small blocks with lanes in stream order, every block
+// coded once (encode_once)
 template <typename S, int D, bool REV>
 int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out, u64 offset, u64* bits_out,
                       hipStream_t st, u64** boffs_out, const Workspace* ws)
@@ -729,15 +920,30 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
     if (index_buffer(held.bytes, &held.p) == ZFP_HIP_OK)
       offs = held.p;
   }
+  bool stream_order = true; // lane t holds stream block t
+  for (int i = 0; i < d.dims; i++)
+    stream_order = stream_order && d.g.lax[i] == (u32)i;
+  // every block coded once when a wave's slots fit LDS (blocks of at most
+  // 32 words); not for the caller's workspace (no allocation, a fixed set of
+  // passes)
+  const u32 bw = (block_bits_bound<S, D, REV>(z) + 63) / 64;
+  if (stream_order && !ws && bw >= 1 && bw <= 32 && d.nblocks > 0) {
+    u64 total = 0;
+    if ((rc = encode_once<S, D, REV>(d, data, z, out, offset, offs, bw, &total, st)))
+      return rc;
+    *bits_out = total;
+    if (boffs_out) {
+      *boffs_out = held.p;
+      held.p = nullptr;
+    }
+    return ZFP_HIP_OK;
+  }
   hipLaunchKernelGGL((k_encode<S, D, ENC_LENGTH, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                      d.nblocks, z, out, offset, (const u64*)nullptr, lens);
   HIPCHK(hipGetLastError());
   rc = ws ? exclusive_scan_tmp(lens, offs, nb1, ws->tmp, st) : exclusive_scan(lens, offs, nb1, st);
   if (rc)
     return rc;
-  bool stream_order = true; // lane t holds stream block t
-  for (int i = 0; i < d.dims; i++)
-    stream_order = stream_order && d.g.lax[i] == (u32)i;
   // a wave's LDS area in the write pass: twice the words 64 average blocks
   // take, at least 64 and at most 2048 (16 KiB).  A wave whose range is
   // longer writes directly, so the size sets the speed, never the stream.

@@ -16,7 +16,7 @@
 
 namespace zfphip {
 
-Scratch g_lens, g_scan_tmp, g_flag;
+Scratch g_lens, g_scan_tmp, g_flag, g_stage;
 
 namespace {
 thread_local char g_last_error[256] = "";
