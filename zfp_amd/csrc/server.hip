@@ -47,6 +47,7 @@ struct Request {
   u64 bits = 0;
   int status = -1;            // -1 while queued or running
   Span rd[2], wr[2];          // memory the request reads and writes
+  u64 cap = 0;                // the job's stream capacity in words (0: not given)
 };
 
 // The field's bytes: element 0 at data, any strides (batch_kernels.h BJob)
@@ -76,10 +77,13 @@ void request_spans(Request& r)
     words = J.nwords;
   else {
     // longest block: header, group tests and maxprec planes, within the
-    // budget (zfp_stream_maximum_size, src/zfp.c:711-742)
+    // budget (zfp_stream_maximum_size, src/zfp.c:711-742); no further than
+    // the capacity the caller gave
     const u64 most = 12 + size - 1 + size * std::min<u64>(J.z.maxprec, P);
     const u64 per = std::max<u64>(J.z.minbits, std::min<u64>(J.z.maxbits, most));
     words = (J.offset + (u64)J.nblocks * per + 63) / 64;
+    if (r.cap)
+      words = std::min<u64>(words, r.cap);
   }
   const uintptr_t w = (uintptr_t)J.words;
   const Span stream{w, w + words * 8};
@@ -186,7 +190,6 @@ struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
-  bool signalled = false;     // set by the stream's host function at the batch's end
   Server* server = nullptr;
   std::vector<Request*> reqs;
   PinBuf<BJob> h_jobs;
@@ -317,14 +320,13 @@ struct Server {
   }
 
   // runs on a runtime thread once slot s's batch (and the copy of its bit
-  // counts) is complete: wakes the scheduler
+  // counts) is complete: wakes the scheduler (taking the lock first, so
+  // that the wake cannot fall between its check and its wait)
   static void batch_done(void* arg)
   {
-    Slot* s = static_cast<Slot*>(arg);
-    Server* sv = s->server;
+    Server* sv = static_cast<Slot*>(arg)->server;
     {
       std::lock_guard<std::mutex> lk(sv->m);
-      s->signalled = true;
     }
     sv->cv_work.notify_all();
   }
@@ -343,7 +345,6 @@ struct Server {
     }
     s.reqs.clear();
     s.busy = false;
-    s.signalled = false;
     cv_done.notify_all();
   }
 
@@ -370,13 +371,13 @@ struct Server {
   {
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
-      // completions: signalled by the streams' host functions; the event
-      // query also catches a batch whose stream failed
+      // completions (the streams' host functions wake this loop; the event
+      // tells, and also catches a batch whose stream failed)
       for (Slot& s : slots) {
         if (!s.busy)
           continue;
         const hipError_t q = hipEventQuery(s.done);
-        if (q == hipSuccess && s.signalled)
+        if (q == hipSuccess)
           finish(s, ZFP_HIP_OK);
         else if (q != hipSuccess && q != hipErrorNotReady) {
           (void)hipGetLastError();
@@ -428,10 +429,10 @@ struct Server {
         busy = busy || s.busy;
       if (stop && pending.empty() && !busy)
         break;
-      // woken by submissions and batch ends; the timeout is only a backstop
-      // for a stream that fails before its host function runs
+      // woken by submissions and by batch ends; the timed wait bounds the
+      // lag of a host function that runs late (or never, on a failed stream)
       if (busy)
-        cv_work.wait_for(lk, std::chrono::milliseconds(5));
+        cv_work.wait_for(lk, std::chrono::microseconds(100));
       else
         cv_work.wait(lk, [this] { return stop || !pending.empty(); });
     }
@@ -541,6 +542,7 @@ extern "C" int zfp_hip_server_submit(zfp_hip_server* server, const zfp_hip_job* 
   }
   r->dir = direction;
   r->bytes = bytes;
+  r->cap = job->nwords;
   request_spans(*r);
   r->t_submit = Clock::now();
   {
