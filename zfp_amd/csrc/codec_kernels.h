@@ -929,14 +929,19 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
   const u32 bw = (block_bits_bound<S, D, REV>(z) + 63) / 64;
   if (stream_order && !ws && bw >= 1 && bw <= 32 && d.nblocks > 0) {
     u64 total = 0;
-    if ((rc = encode_once<S, D, REV>(d, data, z, out, offset, offs, bw, &total, st)))
-      return rc;
-    *bits_out = total;
-    if (boffs_out) {
-      *boffs_out = held.p;
-      held.p = nullptr;
+    rc = encode_once<S, D, REV>(d, data, z, out, offset, offs, bw, &total, st);
+    if (rc == ZFP_HIP_OK) {
+      *bits_out = total;
+      if (boffs_out) {
+        *boffs_out = held.p;
+        held.p = nullptr;
+      }
+      return ZFP_HIP_OK;
     }
-    return ZFP_HIP_OK;
+    // no room for the staged bits (nblocks * bw words): the multi-pass
+    // coder needs only the lengths and offsets
+    if (rc != ZFP_HIP_ENOMEM)
+      return rc;
   }
   hipLaunchKernelGGL((k_encode<S, D, ENC_LENGTH, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
                      d.nblocks, z, out, offset, (const u64*)nullptr, lens);
