@@ -30,7 +30,7 @@ Beside the headline line's own workload, the same JSON line reports:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
        [--rates 4,16,32] [--layouts contiguous,permuted]
-       [--no-cpu] [--no-batch] [--no-serve] [--no-vr]
+       [--no-cpu] [--no-batch] [--no-serve] [--no-vr] [--vr-only CASES]
 """
 import argparse
 import ctypes
@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--layouts", default="contiguous,permuted", help="layouts of the 512^3 passes")
     p.add_argument("--no-vr", action="store_true", help="skip the variable-rate measurement")
     p.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    p.add_argument("--vr-only", default="",
+                   help="variable-rate cases to measure, e.g. 2d_f32_8191x8193_precision16 (comma list; default all)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 1 GiB to 1 GiB three times (the known-size kernel that scales the "
                         "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -390,8 +392,9 @@ def vr_fields(torch):
 
 def vr_bench(args):
     """Variable-rate modes (precision 16/28, accuracy 1e-3/1e-7) on the 2D
-    float32 and 4D float64 arrays: length pass, device scan and write pass
-    per compression; decompression from the recorded block index.  Timed
+    float32 and 4D float64 arrays: each compression codes every block once
+    (2D) or runs a length pass, device scan and write pass (4D);
+    decompression from the recorded block index.  Timed
     per call (wall clock, after a warm-up); checks: decompression consumes
     exactly the bits compression wrote, and the batched kernels (a one-job
     batch, 2D only) write the same stream as the per-field path."""
@@ -404,6 +407,8 @@ def vr_bench(args):
         dims = len(n)
         ubytes = f.numel() * f.element_size()
         for mode, val in modes:
+            if args.vr_only and f"{name}_{mode}{val}" not in args.vr_only.split(","):
+                continue
             p = Z.stream_params(mode, val, ztype, dims)
             nw = Z.max_stream_words(ztype, n, p)
             w = torch.zeros(nw, dtype=torch.int64, device="cuda:0")
