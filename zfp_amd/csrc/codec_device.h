@@ -1133,9 +1133,12 @@ struct PlaneDecoder {
     // and its run take at most 64 bits; they are decoded from a local
     // 64-bit window, which is refilled from the reader only when fewer
     // bits than a test may need remain in it.
+    // The trip is written without branches but the refill: a 0 test
+    // consumes one bit and ends the plane, a 1 test consumes its run too.
     u64 w = r.peek64();
     u32 used = 0;
-    while (budget && n < (u32)N) {
+    bool more = budget && n < (u32)N;
+    while (more) {
       const u32 av = umin((u32)N - 1 - n, budget - 1); // <= 63: scan length
       if (used + umin(av + 2, 64u) > 64) {
         r.skip(used);
@@ -1143,18 +1146,14 @@ struct PlaneDecoder {
         used = 0;
       }
       const u64 wl = w >> used;
-      budget--;
-      if (!(wl & 1)) {
-        used++;
-        break;
-      }
+      const bool g = (wl & 1) != 0;
       const u32 p = ctz64(((wl >> 1) & lowmask(av)) | (1ull << av));
-      const u32 hit = p < av ? 1u : 0u; // the run ends in a read one-bit
-      used += 1 + p + hit;
-      budget -= p + hit;
-      n += p;
-      x |= 1ull << n;
-      n++;
+      const u32 run = g ? p + (p < av ? 1u : 0u) : 0u; // the zeros, and the one-bit that ends them if read
+      used += 1 + run;
+      budget -= 1 + run;
+      x |= g ? 1ull << (n + p) : 0ull;
+      n = g ? n + p + 1 : n;
+      more = g && budget && n < (u32)N;
     }
     r.skip(used);
     return x;
