@@ -198,9 +198,11 @@ int zfp_hip_offsets_from_hybrid8(unsigned dims, const uint64_t* words, size_t nb
 /*
  * Batched codec: no counterpart in the reference, which compresses one field
  * per zfp_compress() call.  Many device-resident fields of mixed type
- * (int32, int64, float, double), dimensionality (1-3), size and mode
- * (fixed rate, precision, accuracy, expert) go through one persistent
- * kernel launch per pass.  The passes are recorded once into a HIP graph
+ * (int32, int64, float, double), dimensionality (1-4), size and mode
+ * (fixed rate, precision, accuracy, expert, reversible) go through one
+ * persistent kernel launch per pass (two when the batch mixes 1-3D lossy
+ * jobs with 4D or reversible ones).  Reversible float and double jobs need
+ * minbits <= 1.  The passes are recorded once into a HIP graph
  * and replayed by each run.  On the first run, the launch shape
  * (workgroup size, waves per CU, blocks per task) is chosen by timing a few
  * candidates; the choice is cached per batch shape.  Each job's stream is
@@ -208,7 +210,7 @@ int zfp_hip_offsets_from_hybrid8(unsigned dims, const uint64_t* words, size_t nb
  */
 typedef struct {
   int type;              /* zfp_type of the field */
-  size_t n[4];           /* extents, 0 for unused dimensions (1-3 dims) */
+  size_t n[4];           /* extents, 0 for unused dimensions */
   ptrdiff_t s[4];        /* strides in elements, 0 for the default */
   void* data;            /* device pointer to element 0 */
   uint minbits, maxbits, maxprec;

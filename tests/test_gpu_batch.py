@@ -1,6 +1,6 @@
 """The batched codec (zfp_hip_batch_*): many fields of mixed type,
-dimensionality, size, layout and mode in one persistent launch per pass,
-replayed from a HIP graph.
+dimensionality (1-4), size, layout and mode (reversible included) in one
+persistent launch per pass and family, replayed from a HIP graph.
 
 Every job's stream must be the oracle's stream for that field bit for bit,
 and every job's decompressed array the oracle's reconstruction, exactly as
@@ -25,6 +25,10 @@ def _torch():
 
 
 def _params(kind, ztype, dims):
+    if kind == "reversible":   # zfp_stream_set_reversible (src/zfp.c:750-757)
+        return (1, 16658, 64, -1075)
+    if kind == "rev_int":      # reversible with a budget and minbits padding (integer types)
+        return (300, 2500, 40, -1075)
     if kind == "rate8":
         return stream_params_rate(8, ztype, dims)
     if kind == "rate16":
@@ -42,12 +46,12 @@ class Case:
     """One job: host field (numpy base + layout), device copies, expected
     stream and reconstruction from the oracle."""
 
-    def __init__(self, rng, k):
+    def __init__(self, rng, k, dims=None, mode=None):
         torch = _torch()
-        dims = int(rng.integers(1, 4))
-        n = [int(v) for v in np.exp(rng.uniform(np.log(3), np.log({1: 400, 2: 40, 3: 14}[dims]), size=dims))]
+        dims = dims or int(rng.integers(1, 4))
+        n = [int(v) for v in np.exp(rng.uniform(np.log(3), np.log({1: 400, 2: 40, 3: 14, 4: 7}[dims]), size=dims))]
         self.ztype = TYPES[k % 4]
-        mode = ["rate8", "rate16", "rate5", "prec20", "acc", "expert"][(k // 4) % 6]
+        mode = mode or ["rate8", "rate16", "rate5", "prec20", "acc", "expert"][(k // 4) % 6]
         self.params = _params(mode, self.ztype, dims)
         self.n = n
         a = smooth_field(n, self.ztype, seed=100 + k, noise=1e-3, special=0.01 if k % 5 == 0 else 0.0)
@@ -60,7 +64,7 @@ class Case:
         self.d_base = torch.from_numpy(np.ascontiguousarray(self.base)).to("cuda:0")
         self.d_words = torch.zeros(self.nwords, dtype=torch.int64, device="cuda:0")
         nb = int(np.prod([(v + 3) // 4 for v in n]))
-        self.var = self.params[0] != self.params[1]
+        self.var = self.params[0] != self.params[1] or self.params[3] < -1074
         self.d_index = torch.zeros(nb + 1, dtype=torch.int64, device="cuda:0") if self.var else None
         self.itemsize = np.dtype(NP_OF_TYPE[self.ztype]).itemsize
         # reconstruction of the oracle's stream, laid out like the field
@@ -84,6 +88,28 @@ def _cases(count, seed):
     return [Case(rng, k) for k in range(count)]
 
 
+def _wide_cases(seed):
+    """4D jobs in the lossy modes, reversible jobs of every dimensionality,
+    and some 1-3D lossy jobs, so that both kernel families run in one
+    batch."""
+    rng = np.random.default_rng(seed)
+    out = []
+    k = 0
+    for mode in ["rate8", "rate5", "prec20", "acc", "expert"]:
+        for _ in range(2):
+            out.append(Case(rng, k, dims=4, mode=mode))
+            k += 1
+    for dims in (1, 2, 3, 4):
+        for _ in range(4):
+            mode = "rev_int" if TYPES[k % 4] in (T_INT32, T_INT64) and k % 8 < 4 else "reversible"
+            out.append(Case(rng, k, dims=dims, mode=mode))
+            k += 1
+    for _ in range(6):
+        out.append(Case(rng, k))
+        k += 1
+    return out
+
+
 def _check_streams(cases, bits):
     for k, (c, b) in enumerate(zip(cases, bits)):
         assert b == c.bits, f"job {k}: {b} bits, oracle {c.bits}"
@@ -93,6 +119,11 @@ def _check_streams(cases, bits):
 @pytest.fixture(scope="module")
 def cases():
     return _cases(48, 7)
+
+
+@pytest.fixture(scope="module")
+def wide_cases():
+    return _wide_cases(9)
 
 
 def test_batch_compress_matches_oracle(cases):
@@ -131,7 +162,7 @@ def test_batch_shapes_give_identical_streams(cases, shape):
     b.close()
 
 
-def test_batch_decompress_matches_oracle(cases):
+def _decompress_and_check(cases):
     torch = _torch()
     comp = Z.Batch([c.job() for c in cases], Z.ZFP_HIP_BATCH_COMPRESS)
     comp.run()
@@ -149,6 +180,35 @@ def test_batch_decompress_matches_oracle(cases):
             ref = read_layout(c.expect_rec, c.off, c.n, c.s)
             assert np.array_equal(mine.view(np.uint8), ref.view(np.uint8)), f"job {k}: reconstruction differs"
     dec.close()
+
+
+def test_batch_decompress_matches_oracle(cases):
+    _decompress_and_check(cases)
+
+
+def test_batch_wide_compress_matches_oracle(wide_cases):
+    """4D and reversible jobs (their own kernel family) beside 1-3D lossy
+    ones, in one batch."""
+    for c in wide_cases:
+        c.d_words.zero_()
+    b = Z.Batch([c.job() for c in wide_cases], Z.ZFP_HIP_BATCH_COMPRESS)
+    for _ in range(2):
+        bits = b.run()
+        _check_streams(wide_cases, bits)
+    b.close()
+
+
+def test_batch_wide_decompress_matches_oracle(wide_cases):
+    _decompress_and_check(wide_cases)
+
+
+def test_batch_wide_reversible_is_lossless(wide_cases):
+    for k, c in enumerate(wide_cases):
+        if c.params != (1, 16658, 64, -1075):   # budgeted reversible jobs may drop planes
+            continue
+        got = read_layout(c.expect_rec, c.off, c.n, c.s)
+        want = read_layout(c.base, c.off, c.n, c.s)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), f"job {k}: reversible job lost bits"
 
 
 def test_batch_matches_per_field_calls(cases):
@@ -172,9 +232,10 @@ def test_batch_rejects_what_it_cannot_run():
     torch = _torch()
     d = torch.zeros(4 * 4 * 4 * 4, dtype=torch.float64, device="cuda:0")
     w = torch.zeros(1024, dtype=torch.int64, device="cuda:0")
-    p = stream_params_rate(8, T_DOUBLE, 4)
-    with pytest.raises(RuntimeError, match="4D"):
-        Z.Batch([Z.make_job(T_DOUBLE, [4, 4, 4, 4], None, d.data_ptr(), p, w.data_ptr())])
+    # reversible float blocks with minbits > 1: serial zfp decodes such
+    # streams only block after block
+    with pytest.raises(RuntimeError, match="minbits"):
+        Z.Batch([Z.make_job(T_DOUBLE, [4, 4, 4, 4], None, d.data_ptr(), (300, 2500, 64, -1075), w.data_ptr())])
     h = np.zeros(64, dtype=np.float64)
     with pytest.raises(RuntimeError, match="device memory"):
         Z.Batch([Z.make_job(T_DOUBLE, [64], None, h.ctypes.data, stream_params_rate(8, T_DOUBLE, 1),

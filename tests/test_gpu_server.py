@@ -24,18 +24,20 @@ def _torch():
 
 
 class Req:
-    """A 3D float field with a mode; its expected stream and reconstruction
-    come from the oracle, and device buffers for compress and decompress
-    requests."""
+    """A float field (3D unless dims says otherwise) with a mode; its
+    expected stream and reconstruction come from the oracle, and device
+    buffers for compress and decompress requests."""
 
-    def __init__(self, rng, k):
+    def __init__(self, rng, k, dims=3, reversible=False):
         torch = _torch()
         self.ztype = [T_FLOAT, T_DOUBLE][k % 2]
-        self.n = [int(v) for v in rng.integers(6, 30, size=3)]
-        if k % 3 == 2:
+        self.n = [int(v) for v in rng.integers(6, 30 if dims < 4 else 11, size=dims)]
+        if reversible:   # zfp_stream_set_reversible (src/zfp.c:750-757)
+            self.params = (1, 16658, 64, -1075)
+        elif k % 3 == 2:
             self.params = stream_params_accuracy(float(10.0 ** rng.uniform(-6, -2)))
         else:
-            self.params = stream_params_rate(int(rng.integers(4, 33)), self.ztype, 3)
+            self.params = stream_params_rate(int(rng.integers(4, 33)), self.ztype, dims)
         a = smooth_field(self.n, self.ztype, seed=500 + k, special=0.01 if k % 4 == 0 else 0.0)
         self.host = a
         self.words, self.bits = oracle_compress(a, 0, self.n, None, self.ztype, self.params)
@@ -46,7 +48,7 @@ class Req:
         self.out = torch.zeros_like(self.field)
         self.w = torch.zeros(self.nwords, dtype=torch.int64, device="cuda:0")
         nb = int(np.prod([(v + 3) // 4 for v in self.n]))
-        self.var = self.params[0] != self.params[1]
+        self.var = self.params[0] != self.params[1] or self.params[3] < -1074
         self.idx = torch.zeros(nb + 1, dtype=torch.int64, device="cuda:0") if self.var else None
         # a stream and index for decompression requests, written up front
         self.w_in = torch.from_numpy(self.words.view(np.int64).copy()).to("cuda:0")
@@ -130,6 +132,31 @@ def test_mixed_requests_match_oracle_and_alone(reqs):
     # alone: one call per job gives the same words and values
     torch = _torch()
     for r, kind, bits, _ in done:
+        w, b1, o, b2 = r.alone()
+        if kind == "c":
+            assert b1 == bits and torch.equal(w[: (bits + 63) // 64], r.w[: (bits + 63) // 64])
+        else:
+            assert b2 == bits and torch.equal(o.view(torch.uint8), r.out.view(torch.uint8))
+
+
+def test_4d_and_reversible_requests_match_oracle_and_alone(reqs):
+    """4D and reversible requests (the batched codec's second kernel family)
+    mixed with 3D lossy ones."""
+    rng = np.random.default_rng(12)
+    wide = [Req(rng, k, dims=4) for k in range(4)]
+    wide += [Req(rng, k, dims=int(d), reversible=True) for k, d in enumerate([1, 2, 3, 4, 3, 4])]
+    # alternate positions, so that wide requests are both compressions
+    # (even places) and decompressions (odd places)
+    mix = []
+    for i, (a, b) in enumerate(zip(wide, reqs)):
+        mix += [a, b] if i % 2 == 0 else [b, a]
+    _reset(mix)
+    sv = Z.Server(64 << 30)
+    done = _serve(mix, sv)
+    sv.close()
+    torch = _torch()
+    for r, kind, bits, _ in done:
+        (r.check_compress if kind == "c" else r.check_decompress)(bits)
         w, b1, o, b2 = r.alone()
         if kind == "c":
             assert b1 == bits and torch.equal(w[: (bits + 63) // 64], r.w[: (bits + 63) // 64])

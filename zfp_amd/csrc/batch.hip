@@ -132,16 +132,20 @@ struct Batch {
     (void)hipGetLastError();
   }
 
-  // tasks of one shape: all jobs (ENC/DEC), then the variable-rate jobs (LEN)
-  void tasks_for(unsigned bpt, std::vector<uint2>& all, std::vector<uint2>& var) const
+  // tasks of one shape, per family (0: 1-3D lossy, 1: 4D or reversible):
+  // all jobs (ENC/DEC), and the variable-rate jobs (LEN)
+  void tasks_for(unsigned bpt, std::vector<uint2>* all, std::vector<uint2>* var) const
   {
-    all.clear();
-    var.clear();
+    for (int f = 0; f < 2; f++) {
+      all[f].clear();
+      var[f].clear();
+    }
     for (u32 j : order) {
+      const u32 fam = jobs[j].wide;
       for (u32 f = 0; f < jobs[j].nblocks; f += bpt) {
-        all.push_back(make_uint2(j, f));
+        all[fam].push_back(make_uint2(j, f));
         if (jobs[j].mode == BM_VAR)
-          var.push_back(make_uint2(j, f));
+          var[fam].push_back(make_uint2(j, f));
       }
     }
   }
@@ -149,9 +153,15 @@ struct Batch {
   // records the passes for shape s into a graph (replacing the current one)
   int capture(const Shape& s)
   {
-    std::vector<uint2> all, var;
+    std::vector<uint2> all[2], var[2];
     tasks_for(s.bpt, all, var);
-    const size_t nt = all.size() + var.size();
+    // d_tasks: all (fast, wide), then var (fast, wide)
+    std::vector<uint2> list;
+    for (auto* v : {&all[0], &all[1], &var[0], &var[1]})
+      list.insert(list.end(), v->begin(), v->end());
+    const size_t nt = list.size();
+    const u32 na0 = (u32)all[0].size(), na1 = (u32)all[1].size();
+    const u32 nv0 = (u32)var[0].size(), nv1 = (u32)var[1].size();
     if (nt > task_cap) {
       if (d_tasks)
         (void)hipFree(d_tasks);
@@ -162,10 +172,8 @@ struct Batch {
         return rc;
       task_cap = nt;
     }
-    if (!all.empty())
-      HIPCHK(hipMemcpy(d_tasks, all.data(), all.size() * sizeof(uint2), hipMemcpyHostToDevice));
-    if (!var.empty())
-      HIPCHK(hipMemcpy(d_tasks + all.size(), var.data(), var.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    if (nt)
+      HIPCHK(hipMemcpy(d_tasks, list.data(), nt * sizeof(uint2), hipMemcpyHostToDevice));
     const unsigned grid = std::max(1u, (unsigned)((u64)ncu * s.wpc * 64 / s.wg));
     const u32 njobs = (u32)jobs.size();
 
@@ -173,18 +181,18 @@ struct Batch {
     int rc = ZFP_HIP_OK;
     if (dir == ZFP_HIP_BATCH_COMPRESS) {
       if (nvar) {
-        rc = launch_batch_pass<BK_LEN>(grid, s.wg, st, d_jobs, d_tasks + all.size(), (u32)var.size(), s.bpt,
-                                       d_lens, d_offs);
+        rc = launch_batch_families<BK_LEN>(grid, s.wg, st, d_jobs, d_tasks + na0 + na1, nv0, nv1, s.bpt, d_lens,
+                                           d_offs);
         if (!rc)
           rc = exclusive_scan_tmp(d_lens, d_offs, nvar + 1, d_tmp, st);
       }
       if (!rc && !prep.empty())
         rc = launch_batch_prepare(ncu * 4, st, d_jobs, d_prep, (u32)prep.size(), d_offs);
       if (!rc)
-        rc = launch_batch_pass<BK_ENC>(grid, s.wg, st, d_jobs, d_tasks, (u32)all.size(), s.bpt, d_lens, d_offs);
+        rc = launch_batch_families<BK_ENC>(grid, s.wg, st, d_jobs, d_tasks, na0, na1, s.bpt, d_lens, d_offs);
     }
     else {
-      rc = launch_batch_pass<BK_DEC>(grid, s.wg, st, d_jobs, d_tasks, (u32)all.size(), s.bpt, d_lens, d_offs);
+      rc = launch_batch_families<BK_DEC>(grid, s.wg, st, d_jobs, d_tasks, na0, na1, s.bpt, d_lens, d_offs);
     }
     if (!rc)
       rc = launch_batch_totals(st, d_jobs, njobs, d_offs, d_bits);
@@ -269,10 +277,16 @@ int job_to_device(const zfp_hip_job& in, int dir, BJob& J, u64& vbase)
   int rc = make_desc(in.type, in.n, in.s, d);
   if (rc)
     return rc;
-  if (d.dims > 3)
-    return record_error(ZFP_HIP_EINVAL, "batch: 4D fields are not batched; use zfp_hip_encode/decode");
-  if (in.minexp < -1074 || in.minbits > in.maxbits || !in.maxbits || !in.maxprec || in.maxprec > 64)
-    return record_error(ZFP_HIP_EINVAL, "batch: invalid codec parameters (reversible mode is not batched)");
+  // reversible mode: minexp below ZFP_MIN_EXP (zfp_stream_set_reversible,
+  // src/zfp.c:750-757)
+  const bool rev = in.minexp < -1074;
+  if (in.minbits > in.maxbits || !in.maxbits || !in.maxprec || in.maxprec > 64)
+    return record_error(ZFP_HIP_EINVAL, "batch: invalid codec parameters");
+  // serial zfp reads an all-zero reversible float block as minbits bits
+  // although one bit was written (revdecodef.c:24-59), so with minbits > 1
+  // such a stream decodes only block after block, never from an index
+  if (rev && in.minbits > 1 && (in.type == zfp_type_float || in.type == zfp_type_double))
+    return record_error(ZFP_HIP_EINVAL, "batch: reversible float jobs need minbits <= 1");
   if (!in.data || !in.stream || !is_device_ptr(in.data) || !is_device_ptr(in.stream))
     return record_error(ZFP_HIP_EINVAL, "batch: field and stream must be device memory");
   J.g = d.g;
@@ -282,8 +296,12 @@ int job_to_device(const zfp_hip_job& in, int dir, BJob& J, u64& vbase)
   J.offset = in.offset;
   J.index = in.index;
   J.nblocks = d.nblocks;
-  J.kind = 3u * (u32)(in.type - 1) + (u32)(d.dims - 1);
-  J.mode = in.minbits == in.maxbits ? BM_FIXED : BM_VAR;
+  J.type = (u32)in.type;
+  J.dims = (u32)d.dims;
+  J.rev = rev ? 1u : 0u;
+  J.wide = rev || d.dims == 4 ? 1u : 0u;
+  J.kind = job_kind(J.type, J.dims, rev);
+  J.mode = !rev && in.minbits == in.maxbits ? BM_FIXED : BM_VAR;
   J.dir = dir == ZFP_HIP_BATCH_DECOMPRESS ? 1u : 0u;
   if (J.mode == BM_VAR && dir == ZFP_HIP_BATCH_COMPRESS) {
     J.vbase = vbase;
@@ -306,19 +324,22 @@ namespace {
 
 std::string signature(const Batch& b)
 {
-  // direction, and per kind the job count and log2 of its block total
-  u64 blocks[12] = {0};
-  unsigned count[12] = {0};
+  // direction, and per kind of either family the job count and log2 of
+  // its block total
+  constexpr int NK = 12 + 20;
+  u64 blocks[NK] = {0};
+  unsigned count[NK] = {0};
   bool var = false;
   for (const BJob& J : b.jobs) {
-    blocks[J.kind] += J.nblocks;
-    count[J.kind]++;
+    const u32 k = J.wide ? 12 + J.kind : J.kind;
+    blocks[k] += J.nblocks;
+    count[k]++;
     var = var || J.mode == BM_VAR;
   }
   std::string s = b.dir ? "D" : "C";
   s += var ? "v" : "f";
   char buf[32];
-  for (int k = 0; k < 12; k++) {
+  for (int k = 0; k < NK; k++) {
     int lg = 0;
     while ((blocks[k] >> lg) > 1)
       lg++;
@@ -354,8 +375,10 @@ int create(const zfp_hip_job* in, size_t njobs, int dir, void* user_stream, Batc
     if (dir == ZFP_HIP_BATCH_COMPRESS && (J.mode == BM_VAR || (J.offset & 63) || (J.z.maxbits & 63)))
       b->prep.push_back(j);
   }
-  std::stable_sort(b->order.begin(), b->order.end(),
-                   [b](u32 x, u32 y) { return b->jobs[x].kind < b->jobs[y].kind; });
+  std::stable_sort(b->order.begin(), b->order.end(), [b](u32 x, u32 y) {
+    const BJob &a = b->jobs[x], &c = b->jobs[y];
+    return a.wide != c.wide ? a.wide < c.wide : a.kind < c.kind;
+  });
   int dev = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)

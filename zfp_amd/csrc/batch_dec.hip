@@ -1,7 +1,7 @@
-// batch_dec.hip -- instantiation of the batched codec's DEC pass.
+// batch_dec.hip -- instantiation of the batched codec's DEC pass for 1-3D
+// lossy jobs.
 #include "batch_kernels.h"
 
 namespace zfphip {
-template int launch_batch_pass<BK_DEC>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32, u32, u64*,
-                                   const u64*);
+template ZH_BATCH_PASS(BK_DEC, false);
 } // namespace zfphip

@@ -1,7 +1,7 @@
-// batch_enc.hip -- instantiation of the batched codec's ENC pass.
+// batch_enc.hip -- instantiation of the batched codec's ENC pass for 1-3D
+// lossy jobs.
 #include "batch_kernels.h"
 
 namespace zfphip {
-template int launch_batch_pass<BK_ENC>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32, u32, u64*,
-                                   const u64*);
+template ZH_BATCH_PASS(BK_ENC, false);
 } // namespace zfphip

@@ -1,6 +1,7 @@
 // batch_kernels.h -- device side of the batched codec: many fields of mixed
-// type (int32, int64, float, double), dimensionality (1-3), size and mode go
-// through one persistent launch per pass.
+// type (int32, int64, float, double), dimensionality (1-4), size and mode
+// (reversible included) go through persistent launches, one per pass and
+// family.
 //
 // Work is a list of tasks (job, first lane); each wave takes tasks in turn
 // (task = wave id, + number of waves, ...) and runs up to blocks_per_task
@@ -8,6 +9,12 @@
 // per-block codec as the single-field kernels.  The job's kind selects the
 // codec instantiation; it is uniform across the wave, so the switch costs a
 // scalar branch.
+//
+// Families: 1-3D lossy jobs (12 kinds) run in one kernel; 4D and reversible
+// jobs (20 kinds: 4D lossy per type, reversible per type and
+// dimensionality) in a second.  The second family's blocks keep large
+// lane-private arrays, whose registers and scratch would otherwise be the
+// first family's too, since a kernel is allocated for its largest case.
 //
 // Passes: LEN (block lengths of variable-rate jobs), then an exclusive scan
 // of those lengths, ENC (every job writes its blocks: fixed rate at
@@ -37,12 +44,24 @@ struct BJob {
   u64 vbase;     // slot of the first block in the batch's length/offset arrays
   u64 nwords;    // readable words of `words` (decompression bound)
   u32 nblocks;
-  u32 kind;      // 3 * (zfp_type - 1) + (dims - 1)
-  u32 mode;      // BM_FIXED or BM_VAR
+  u32 kind;      // codec within the family (job_kind)
+  u32 mode;      // BM_FIXED or BM_VAR (reversible jobs are variable-rate)
   u32 dir;       // 0 compression, 1 decompression
+  u32 type;      // zfp_type (1 int32, 2 int64, 3 float, 4 double)
+  u32 dims;      // 1-4
+  u32 wide;      // family: 0 for 1-3D lossy, 1 for 4D or reversible
+  u32 rev;       // reversible mode
 };
 
-template <typename S, int D, int PASS>
+// the kind of a job within its family
+inline u32 job_kind(u32 type, u32 dims, bool rev)
+{
+  if (!rev && dims < 4)
+    return 3u * (type - 1) + (dims - 1);
+  return rev ? 4u + 4u * (type - 1) + (dims - 1) : type - 1;
+}
+
+template <typename S, int D, int PASS, bool REV = false>
 __device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const u64* offs)
 {
   u32 bc[4] = {0, 0, 0, 0};
@@ -50,7 +69,10 @@ __device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const
   if constexpr (PASS == BK_LEN) {
     BitCounter c;
     c.pos = 0;
-    lens[J.vbase + b] = encode_block<S, D>(c, (const S*)J.data, J.g, bc, J.z);
+    if constexpr (REV)
+      lens[J.vbase + b] = encode_block_rev<S, D>(c, (const S*)J.data, J.g, bc, J.z);
+    else
+      lens[J.vbase + b] = encode_block<S, D>(c, (const S*)J.data, J.g, bc, J.z);
   }
   else if constexpr (PASS == BK_ENC) {
     u64 s0, s1;
@@ -67,20 +89,52 @@ __device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const
     }
     BitWriter<false> w;
     w.begin(J.words, J.offset + s0, J.offset + s1);
-    encode_block<S, D>(w, (const S*)J.data, J.g, bc, J.z);
+    if constexpr (REV)
+      encode_block_rev<S, D>(w, (const S*)J.data, J.g, bc, J.z);
+    else
+      encode_block<S, D>(w, (const S*)J.data, J.g, bc, J.z);
     w.finish();
   }
   else {
     const u64 s0 = J.mode == BM_FIXED ? (u64)b * J.z.maxbits : J.index[b];
     BitReader r;
     r.begin(J.words, J.nwords, J.offset + s0);
-    decode_block<S, D, true>(r, (S*)J.data, J.g, bc, J.z);
+    if constexpr (REV)
+      decode_block_rev<S, D, true>(r, (S*)J.data, J.g, bc, J.z);
+    else
+      decode_block<S, D, true>(r, (S*)J.data, J.g, bc, J.z);
   }
 }
 
-template <int PASS>
+// 4D lossy (kinds 0-3, by type) and reversible (4 + 4 * (type - 1) +
+// dims - 1) jobs
+template <int PASS, typename S>
+__device__ __forceinline__ void batch_wide_type(const BJob& J, u32 t, u64* lens, const u64* offs)
+{
+  if (!J.rev) {
+    batch_one<S, 4, PASS>(J, t, lens, offs);
+    return;
+  }
+  switch (J.dims) {
+    case 1: batch_one<S, 1, PASS, true>(J, t, lens, offs); break;
+    case 2: batch_one<S, 2, PASS, true>(J, t, lens, offs); break;
+    case 3: batch_one<S, 3, PASS, true>(J, t, lens, offs); break;
+    default: batch_one<S, 4, PASS, true>(J, t, lens, offs); break;
+  }
+}
+
+template <int PASS, bool WIDE>
 __device__ __forceinline__ void batch_dispatch(const BJob& J, u32 t, u64* lens, const u64* offs)
 {
+  if constexpr (WIDE) {
+    switch (J.type) {
+      case 1: batch_wide_type<PASS, i32>(J, t, lens, offs); break;
+      case 2: batch_wide_type<PASS, i64>(J, t, lens, offs); break;
+      case 3: batch_wide_type<PASS, float>(J, t, lens, offs); break;
+      default: batch_wide_type<PASS, double>(J, t, lens, offs); break;
+    }
+    return;
+  }
   switch (J.kind) {
     case 0: batch_one<i32, 1, PASS>(J, t, lens, offs); break;
     case 1: batch_one<i32, 2, PASS>(J, t, lens, offs); break;
@@ -97,8 +151,8 @@ __device__ __forceinline__ void batch_dispatch(const BJob& J, u32 t, u64* lens, 
   }
 }
 
-// persistent kernel of one pass; tasks[i] = {job, first lane}
-template <int PASS>
+// persistent kernel of one pass and family; tasks[i] = {job, first lane}
+template <int PASS, bool WIDE>
 __global__ void __launch_bounds__(256, 2)
 k_batch(const BJob* __restrict__ jobs, const uint2* __restrict__ tasks, u32 ntasks, u32 per_task,
         u64* __restrict__ lens, const u64* __restrict__ offs)
@@ -113,19 +167,20 @@ k_batch(const BJob* __restrict__ jobs, const uint2* __restrict__ tasks, u32 ntas
     for (u32 t0 = tk.y; t0 < end; t0 += 64) {
       const u32 t = t0 + lane;
       if (t < end)
-        batch_dispatch<PASS>(J, t, lens, offs);
+        batch_dispatch<PASS, WIDE>(J, t, lens, offs);
     }
   }
 }
 
-// host side: launch one pass.  Each pass is instantiated in its own
-// translation unit (batch_len.hip, batch_enc.hip, batch_dec.hip) so that the
-// three compile in parallel.
-template <int PASS>
+// host side: launch one pass of one family.  Each pass and family is
+// instantiated in its own translation unit (batch_{len,enc,dec}.hip and
+// batch_wide_{len,enc,dec}.hip) so that they compile in parallel.
+template <int PASS, bool WIDE>
 int launch_batch_pass(unsigned grid, unsigned wg, hipStream_t st, const BJob* jobs, const uint2* tasks,
                       u32 ntasks, u32 per_task, u64* lens, const u64* offs)
 {
-  hipLaunchKernelGGL((k_batch<PASS>), dim3(grid), dim3(wg), 0, st, jobs, tasks, ntasks, per_task, lens, offs);
+  hipLaunchKernelGGL((k_batch<PASS, WIDE>), dim3(grid), dim3(wg), 0, st, jobs, tasks, ntasks, per_task, lens,
+                     offs);
   HIPCHK(hipGetLastError());
   return ZFP_HIP_OK;
 }
@@ -142,11 +197,28 @@ int launch_batch_prepare(unsigned grid, hipStream_t st, const BJob* jobs, const 
 // jobs) by job j; completes the index of variable-rate compression jobs
 int launch_batch_totals(hipStream_t st, const BJob* jobs, u32 njobs, const u64* offs, u64* bits);
 
-extern template int launch_batch_pass<BK_LEN>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32,
-                                              u32, u64*, const u64*);
-extern template int launch_batch_pass<BK_ENC>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32,
-                                              u32, u64*, const u64*);
-extern template int launch_batch_pass<BK_DEC>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32,
-                                              u32, u64*, const u64*);
+#define ZH_BATCH_PASS(P, W)                                                                                \
+  int launch_batch_pass<P, W>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32, u32, u64*,       \
+                              const u64*)
+extern template ZH_BATCH_PASS(BK_LEN, false);
+extern template ZH_BATCH_PASS(BK_ENC, false);
+extern template ZH_BATCH_PASS(BK_DEC, false);
+extern template ZH_BATCH_PASS(BK_LEN, true);
+extern template ZH_BATCH_PASS(BK_ENC, true);
+extern template ZH_BATCH_PASS(BK_DEC, true);
+
+// both families' launches of one pass over task lists split by family
+// (tasks[0, nfast) fast, then nwide wide tasks)
+template <int PASS>
+int launch_batch_families(unsigned grid, unsigned wg, hipStream_t st, const BJob* jobs, const uint2* tasks,
+                          u32 nfast, u32 nwide, u32 per_task, u64* lens, const u64* offs)
+{
+  int rc = ZFP_HIP_OK;
+  if (nfast)
+    rc = launch_batch_pass<PASS, false>(grid, wg, st, jobs, tasks, nfast, per_task, lens, offs);
+  if (!rc && nwide)
+    rc = launch_batch_pass<PASS, true>(grid, wg, st, jobs, tasks + nfast, nwide, per_task, lens, offs);
+  return rc;
+}
 
 } // namespace zfphip

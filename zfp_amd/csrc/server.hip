@@ -53,8 +53,7 @@ struct Request {
 // The field's bytes: element 0 at data, any strides (batch_kernels.h BJob)
 Span field_span(const BJob& J)
 {
-  const u32 type = J.kind / 3 + 1;
-  const i64 es = type == 1 || type == 3 ? 4 : 8;
+  const i64 es = J.type == 1 || J.type == 3 ? 4 : 8;
   i64 lo = 0, hi = 0;
   for (int d = 0; d < 4; d++) {
     const i64 e = (i64)(J.g.n[d] ? J.g.n[d] - 1 : 0) * J.g.s[d];
@@ -70,8 +69,10 @@ Span field_span(const BJob& J)
 void request_spans(Request& r)
 {
   const BJob& J = r.job;
-  const u32 dims = J.kind % 3 + 1, type = J.kind / 3 + 1;
-  const u64 size = 1ull << (2 * dims), P = type == 1 || type == 3 ? 32 : 64;
+  const u64 size = 1ull << (2 * J.dims), P = J.type == 1 || J.type == 3 ? 32 : 64;
+  // block header bits by type, lossy and reversible (int32, int64, float,
+  // double)
+  static const u64 hdr[2][4] = {{0, 0, 9, 12}, {5, 6, 15, 19}};
   u64 words;
   if (r.dir == ZFP_HIP_BATCH_DECOMPRESS)
     words = J.nwords;
@@ -79,7 +80,7 @@ void request_spans(Request& r)
     // longest block: header, group tests and maxprec planes, within the
     // budget (zfp_stream_maximum_size, src/zfp.c:711-742); no further than
     // the capacity the caller gave
-    const u64 most = 12 + size - 1 + size * std::min<u64>(J.z.maxprec, P);
+    const u64 most = hdr[J.rev][J.type - 1] + size - 1 + size * std::min<u64>(J.z.maxprec, P);
     const u64 per = std::max<u64>(J.z.minbits, std::min<u64>(J.z.maxbits, most));
     words = (J.offset + (u64)J.nblocks * per + 63) / 64;
     if (r.cap)
@@ -226,7 +227,8 @@ struct Server {
   int launch(Slot& s, const std::vector<Request*>& batch)
   {
     const size_t n = batch.size();
-    std::vector<uint2> comp_all, comp_var, dec_all;
+    // task lists per family (0: 1-3D lossy, 1: 4D or reversible)
+    std::vector<uint2> comp_all[2], comp_var[2], dec_all[2];
     std::vector<u32> prep;
     u64 vbase = 0;
     int rc = s.h_jobs.ensure(n);
@@ -250,16 +252,23 @@ struct Server {
       for (u32 f = 0; f < J.nblocks; f += SRV_BPT) {
         const uint2 t = make_uint2((u32)i, f);
         if (!comp)
-          dec_all.push_back(t);
+          dec_all[J.wide].push_back(t);
         else {
-          comp_all.push_back(t);
+          comp_all[J.wide].push_back(t);
           if (J.mode == BM_VAR)
-            comp_var.push_back(t);
+            comp_var[J.wide].push_back(t);
         }
       }
       s.h_jobs.p[i] = J;
     }
-    const size_t nt = comp_all.size() + comp_var.size() + dec_all.size();
+    // d_tasks: comp_all, comp_var, dec_all, each fast then wide
+    std::vector<uint2> list;
+    for (auto* v : {&comp_all[0], &comp_all[1], &comp_var[0], &comp_var[1], &dec_all[0], &dec_all[1]})
+      list.insert(list.end(), v->begin(), v->end());
+    const size_t nt = list.size();
+    const u32 nc0 = (u32)comp_all[0].size(), nc1 = (u32)comp_all[1].size();
+    const u32 nv0 = (u32)comp_var[0].size(), nv1 = (u32)comp_var[1].size();
+    const u32 nd0 = (u32)dec_all[0].size(), nd1 = (u32)dec_all[1].size();
     rc = s.h_tasks.ensure(nt);
     if (!rc)
       rc = s.d_tasks.ensure(nt);
@@ -277,13 +286,11 @@ struct Server {
     }
     if (rc)
       return rc;
-    std::copy(comp_all.begin(), comp_all.end(), s.h_tasks.p);
-    std::copy(comp_var.begin(), comp_var.end(), s.h_tasks.p + comp_all.size());
-    std::copy(dec_all.begin(), dec_all.end(), s.h_tasks.p + comp_all.size() + comp_var.size());
+    std::copy(list.begin(), list.end(), s.h_tasks.p);
     std::copy(prep.begin(), prep.end(), s.h_prep.p);
     const uint2* t_comp = s.d_tasks.p;
-    const uint2* t_var = t_comp + comp_all.size();
-    const uint2* t_dec = t_var + comp_var.size();
+    const uint2* t_var = t_comp + nc0 + nc1;
+    const uint2* t_dec = t_var + nv0 + nv1;
     const unsigned grid = std::max(1u, (unsigned)((u64)ncu * SRV_WPC * 64 / SRV_WG));
 
     for (Request* r : batch)
@@ -296,19 +303,19 @@ struct Server {
       HIPCHK(hipMemcpyAsync(s.d_prep.p, s.h_prep.p, prep.size() * sizeof(u32), hipMemcpyHostToDevice, s.st));
     if (vbase) {
       HIPCHK(hipMemsetAsync(s.d_lens.p + vbase, 0, sizeof(u64), s.st));
-      rc = launch_batch_pass<BK_LEN>(grid, SRV_WG, s.st, s.d_jobs.p, t_var, (u32)comp_var.size(), SRV_BPT,
-                                     s.d_lens.p, s.d_offs.p);
+      rc = launch_batch_families<BK_LEN>(grid, SRV_WG, s.st, s.d_jobs.p, t_var, nv0, nv1, SRV_BPT, s.d_lens.p,
+                                         s.d_offs.p);
       if (!rc)
         rc = exclusive_scan_tmp(s.d_lens.p, s.d_offs.p, vbase + 1, s.d_tmp.p, s.st);
     }
     if (!rc && !prep.empty())
       rc = launch_batch_prepare(ncu * 4, s.st, s.d_jobs.p, s.d_prep.p, (u32)prep.size(), s.d_offs.p);
-    if (!rc && !comp_all.empty())
-      rc = launch_batch_pass<BK_ENC>(grid, SRV_WG, s.st, s.d_jobs.p, t_comp, (u32)comp_all.size(), SRV_BPT,
-                                     s.d_lens.p, s.d_offs.p);
-    if (!rc && !dec_all.empty())
-      rc = launch_batch_pass<BK_DEC>(grid, SRV_WG, s.st, s.d_jobs.p, t_dec, (u32)dec_all.size(), SRV_BPT,
-                                     s.d_lens.p, s.d_offs.p);
+    if (!rc)
+      rc = launch_batch_families<BK_ENC>(grid, SRV_WG, s.st, s.d_jobs.p, t_comp, nc0, nc1, SRV_BPT, s.d_lens.p,
+                                         s.d_offs.p);
+    if (!rc)
+      rc = launch_batch_families<BK_DEC>(grid, SRV_WG, s.st, s.d_jobs.p, t_dec, nd0, nd1, SRV_BPT, s.d_lens.p,
+                                         s.d_offs.p);
     if (!rc)
       rc = launch_batch_totals(s.st, s.d_jobs.p, (u32)n, s.d_offs.p, s.d_bits.p);
     if (rc)
