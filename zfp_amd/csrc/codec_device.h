@@ -1111,32 +1111,33 @@ struct PlaneDecoder {
     // that follows them is a 0 (the usual end of a plane) it is taken in
     // the same step
     const u32 m = umin(n, budget);
-    u64 x;
-    if (m < 64) {
-      const u64 v = r.peek64();
-      x = v & lowmask(m);
-      const bool stop = n < (u32)N && budget > m && !((v >> m) & 1);
-      const u32 take = m + (stop ? 1u : 0u);
-      r.skip(take);
-      budget -= take;
-      if (stop)
-        return x;
-    }
-    else {
-      x = r.peek64();
+    if (m >= 64) {
+      // all 64 coefficients significant: the plane verbatim, no group test
+      const u64 x = r.peek64();
       r.skip(64);
       budget -= 64;
+      return x;
+    }
+    const u64 v = r.peek64();
+    u64 x = v & lowmask(m);
+    const bool stop = n < (u32)N && budget > m && !((v >> m) & 1);
+    const u32 take = m + (stop ? 1u : 0u);
+    budget -= take;
+    if (stop) {
+      r.skip(take);
+      return x;
     }
     // group tests: a 1, then the run of zeros up to the next one-bit; a
     // scan that reaches the last coefficient or the end of the budget
     // stops there and that coefficient takes the one-bit.  One group test
     // and its run take at most 64 bits; they are decoded from a local
-    // 64-bit window, which is refilled from the reader only when fewer
-    // bits than a test may need remain in it.
+    // 64-bit window (first the verbatim step's, past its `take` bits),
+    // which is refilled from the reader only when fewer bits than a test
+    // may need remain in it.
     // The trip is written without branches but the refill: a 0 test
     // consumes one bit and ends the plane, a 1 test consumes its run too.
-    u64 w = r.peek64();
-    u32 used = 0;
+    u64 w = v;
+    u32 used = take;
     bool more = budget && n < (u32)N;
     while (more) {
       const u32 av = umin((u32)N - 1 - n, budget - 1); // <= 63: scan length
