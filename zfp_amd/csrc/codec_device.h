@@ -101,6 +101,8 @@ ZH_INLINE u64 lowmask(u32 n) { return n >= 64 ? ~0ull : ((1ull << n) - 1); }
 ZH_INLINE u64 shr64(u64 x, u32 n) { return n >= 64 ? 0ull : x >> n; }
 ZH_INLINE u64 shl64(u64 x, u32 n) { return n >= 64 ? 0ull : x << n; }
 ZH_INLINE u32 ctz64(u64 x) { return (u32)__builtin_ctzll(x); }
+// trailing zeros of x, 64 when x is 0
+ZH_INLINE u32 ctz64z(u64 x) { return x ? (u32)__builtin_ctzll(x) : 64u; }
 ZH_INLINE u32 umin(u32 a, u32 b) { return a < b ? a : b; }
 // true when the predicate holds in any active lane of the wave
 ZH_INLINE bool wave_any(bool p) { return __any(p) != 0; }
@@ -1147,13 +1149,16 @@ struct PlaneDecoder {
         used = 0;
       }
       const u64 wl = w >> used;
-      const bool g = (wl & 1) != 0;
-      const u32 p = ctz64(((wl >> 1) & lowmask(av)) | (1ull << av));
-      const u32 run = g ? p + (p < av ? 1u : 0u) : 0u; // the zeros, and the one-bit that ends them if read
-      used += 1 + run;
-      budget -= 1 + run;
-      x |= g ? 1ull << (n + p) : 0ull;
-      n = g ? n + p + 1 : n;
+      const u32 g = (u32)wl & 1u;
+      // p1 - 1: the zeros after the test bit, at most av (a scan that
+      // reaches its end takes the coefficient there)
+      const u32 p1 = umin(ctz64z(wl & ~1ull), av + 1);
+      // the test bit, the zeros, and the one-bit that ends them if read
+      const u32 step = g ? p1 + (p1 <= av ? 1u : 0u) : 1u;
+      used += step;
+      budget -= step;
+      x |= (u64)g << (n + p1 - 1); // n + p1 - 1 <= N - 1
+      n += g ? p1 : 0u;
       more = g && budget && n < (u32)N;
     }
     r.skip(used);
