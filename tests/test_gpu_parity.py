@@ -98,6 +98,23 @@ def test_modes_types_shapes(shape, tname, mode):
     check(a, n, ztype, mode_params(mode, ztype, len(n)))
 
 
+@pytest.mark.parametrize("rate", [17, 24, 32, 48])
+@pytest.mark.parametrize("tname", list(TYPES))
+@pytest.mark.parametrize("layout", ["as_is", "permuted_neg"])
+def test_long_fixed_rate_blocks(layout, tname, rate):
+    """3D fixed-rate blocks of more than 16 words, word-aligned, every type:
+    the slot coders with their longest slots (the wave's blocks copied as
+    one run of words).  300 blocks, so five waves, the last one partial, and
+    partial blocks on every axis."""
+    n = [37, 22, 19]
+    ztype = TYPES[tname]
+    a = smooth_field(n, ztype, seed=zlib.crc32(f"long/{tname}/{rate}".encode()) & 0xffff,
+                     special=0.02 if ztype in (T_FLOAT, T_DOUBLE) else 0.0)
+    p = stream_params_rate(rate, ztype, 3)
+    assert p[1] % 64 == 0 and p[1] > 16 * 64
+    check(a, n, ztype, p, layout=layout)
+
+
 @pytest.mark.parametrize("layout", ["reversed", "interleaved", "permuted", "permuted_neg"])
 @pytest.mark.parametrize("mode", ["rate8", "prec16", "acc1e-3"])
 @pytest.mark.parametrize("tname", ["f32", "f64", "i64"])
