@@ -17,19 +17,38 @@ namespace zfphip {
 
 // Zeroes the stream words that blocks of the listed jobs OR into (all words
 // of their ranges; the first word keeps the bits below the job's offset).
-// Every thread of the grid takes a share of every job.
-__global__ void k_batch_prepare(const BJob* __restrict__ jobs, const u32* __restrict__ pjobs, u32 npjobs,
-                                const u64* __restrict__ offs)
+// Every thread of the grid takes a share of every job.  The jobs' word
+// ranges are read in parallel into LDS, PREP_CHUNK jobs at a time, so that
+// the walk over the jobs waits on LDS, not on two dependent global loads
+// per job.
+constexpr u32 PREP_CHUNK = 256;
+
+__global__ void __launch_bounds__(256) k_batch_prepare(const BJob* __restrict__ jobs, const u32* __restrict__ pjobs,
+                                                       u32 npjobs, const u64* __restrict__ offs)
 {
+  __shared__ u64* s_words[PREP_CHUNK];
+  __shared__ u64 s_w0[PREP_CHUNK], s_w1[PREP_CHUNK];
   const u64 gt = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   const u64 gn = (u64)gridDim.x * blockDim.x;
-  for (u32 k = 0; k < npjobs; k++) {
-    const BJob& J = jobs[pjobs[k]];
-    const u64 bits = J.mode == BM_FIXED ? (u64)J.nblocks * J.z.maxbits : offs[J.vbase + J.nblocks] - offs[J.vbase];
-    const u64 w0 = J.offset >> 6, w1 = (J.offset + bits + 63) >> 6;
-    const u32 keep = (u32)(J.offset & 63);
-    for (u64 i = w0 + gt; i < w1; i += gn)
-      J.words[i] = (i == w0 && keep) ? (J.words[i] & lowmask(keep)) : 0ull;
+  for (u32 base = 0; base < npjobs; base += PREP_CHUNK) {
+    const u32 cnt = umin(PREP_CHUNK, npjobs - base);
+    for (u32 i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const BJob& J = jobs[pjobs[base + i]];
+      const u64 bits =
+          J.mode == BM_FIXED ? (u64)J.nblocks * J.z.maxbits : offs[J.vbase + J.nblocks] - offs[J.vbase];
+      s_words[i] = J.words;
+      s_w0[i] = J.offset; // bit offset; the word range follows below
+      s_w1[i] = (J.offset + bits + 63) >> 6;
+    }
+    __syncthreads();
+    for (u32 k = 0; k < cnt; k++) {
+      u64* words = s_words[k];
+      const u64 off = s_w0[k], w0 = off >> 6, w1 = s_w1[k];
+      const u32 keep = (u32)(off & 63);
+      for (u64 i = w0 + gt; i < w1; i += gn)
+        words[i] = (i == w0 && keep) ? (words[i] & lowmask(keep)) : 0ull;
+    }
+    __syncthreads();
   }
 }
 
