@@ -211,7 +211,8 @@ struct Batch {
         rc = launch_batch_families<BK_ENC>(grid, s.wg, st, d_jobs, d_tasks, na0, na1, s.bpt, d_lens, d_offs);
     }
     else {
-      rc = launch_batch_families<BK_DEC>(grid, s.wg, st, d_jobs, d_tasks, na0, na1, s.bpt, d_lens, d_offs);
+      rc = launch_batch_families<BK_DEC>(grid, s.wg, st, d_jobs, d_tasks, na0, na1, s.bpt, d_lens, d_offs,
+                                         batch_stage_stride(jobs.data(), jobs.size()));
     }
     if (!rc)
       rc = launch_batch_totals(st, d_jobs, njobs, d_offs, d_bits);
@@ -335,6 +336,14 @@ int job_to_device(const zfp_hip_job& in, int dir, BJob& J, u64& vbase)
     // is known, which bounds a zero capacity too
     const u64 need = (in.offset + (u64)d.nblocks * in.maxbits + 63) / 64;
     J.nwords = in.nwords ? in.nwords : need;
+    // fixed rate, word-aligned blocks in stream order: staged through LDS
+    bool order = true;
+    for (int i = 0; i < d.dims; i++)
+      order = order && d.g.lax[i] == (u32)i;
+    J.stage = J.mode == BM_FIXED && !J.wide && order && !(in.offset & 63) && !(in.maxbits & 63) &&
+                      in.maxbits / 64 <= BATCH_STAGE_WORDS && J.nwords > 0
+                  ? 1u
+                  : 0u;
   }
   return ZFP_HIP_OK;
 }

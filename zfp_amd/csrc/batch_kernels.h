@@ -19,7 +19,10 @@
 // Passes: LEN (block lengths of variable-rate jobs), then an exclusive scan
 // of those lengths, ENC (every job writes its blocks: fixed rate at
 // b * maxbits, variable rate at its scanned offset) and, for decompression,
-// DEC.
+// DEC.  DEC stages the blocks of fixed-rate jobs with word-aligned blocks in
+// stream order through LDS, one slot per lane, as the single-field decoder
+// does: the wave's 64 blocks are one run of stream words, copied with
+// coalesced loads, and the decoder then reads its slot by bit index.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,6 +30,7 @@
 #include "backend.h"
 #include "codec_block.h"
 #include "codec_device.h"
+#include "codec_kernels.h"
 
 namespace zfphip {
 
@@ -51,7 +55,14 @@ struct BJob {
   u32 dims;      // 1-4
   u32 wide;      // family: 0 for 1-3D lossy, 1 for 4D or reversible
   u32 rev;       // reversible mode
+  u32 stage;     // DEC stages the blocks through LDS slots (fixed rate,
+                 // word-aligned blocks of at most BATCH_STAGE_WORDS words,
+                 // lanes in stream order)
 };
+
+// the longest blocks, in words, that DEC stages through LDS: a workgroup of
+// 256 lanes then needs at most 256 x 17 words (34 KiB)
+constexpr u32 BATCH_STAGE_WORDS = 16;
 
 // the kind of a job within its family
 inline u32 job_kind(u32 type, u32 dims, bool rev)
@@ -61,8 +72,9 @@ inline u32 job_kind(u32 type, u32 dims, bool rev)
   return rev ? 4u + 4u * (type - 1) + (dims - 1) : type - 1;
 }
 
+// slot: the lane's LDS slot holding its block (staged DEC jobs), else null
 template <typename S, int D, int PASS, bool REV = false>
-__device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const u64* offs)
+__device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const u64* offs, const u64* slot)
 {
   u32 bc[4] = {0, 0, 0, 0};
   const u32 b = lane_block<D>(t, J.g, bc);
@@ -95,6 +107,11 @@ __device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const
       encode_block<S, D>(w, (const S*)J.data, J.g, bc, J.z);
     w.finish();
   }
+  else if (!REV && slot) {
+    SlotReader r;
+    r.begin(slot, 0);
+    decode_block<S, D, true>(r, (S*)J.data, J.g, bc, J.z);
+  }
   else {
     const u64 s0 = J.mode == BM_FIXED ? (u64)b * J.z.maxbits : J.index[b];
     BitReader r;
@@ -112,19 +129,19 @@ template <int PASS, typename S>
 __device__ __forceinline__ void batch_wide_type(const BJob& J, u32 t, u64* lens, const u64* offs)
 {
   if (!J.rev) {
-    batch_one<S, 4, PASS>(J, t, lens, offs);
+    batch_one<S, 4, PASS>(J, t, lens, offs, nullptr);
     return;
   }
   switch (J.dims) {
-    case 1: batch_one<S, 1, PASS, true>(J, t, lens, offs); break;
-    case 2: batch_one<S, 2, PASS, true>(J, t, lens, offs); break;
-    case 3: batch_one<S, 3, PASS, true>(J, t, lens, offs); break;
-    default: batch_one<S, 4, PASS, true>(J, t, lens, offs); break;
+    case 1: batch_one<S, 1, PASS, true>(J, t, lens, offs, nullptr); break;
+    case 2: batch_one<S, 2, PASS, true>(J, t, lens, offs, nullptr); break;
+    case 3: batch_one<S, 3, PASS, true>(J, t, lens, offs, nullptr); break;
+    default: batch_one<S, 4, PASS, true>(J, t, lens, offs, nullptr); break;
   }
 }
 
 template <int PASS, bool WIDE>
-__device__ __forceinline__ void batch_dispatch(const BJob& J, u32 t, u64* lens, const u64* offs)
+__device__ __forceinline__ void batch_dispatch(const BJob& J, u32 t, u64* lens, const u64* offs, const u64* slot)
 {
   if constexpr (WIDE) {
     switch (J.type) {
@@ -136,27 +153,30 @@ __device__ __forceinline__ void batch_dispatch(const BJob& J, u32 t, u64* lens, 
     return;
   }
   switch (J.kind) {
-    case 0: batch_one<i32, 1, PASS>(J, t, lens, offs); break;
-    case 1: batch_one<i32, 2, PASS>(J, t, lens, offs); break;
-    case 2: batch_one<i32, 3, PASS>(J, t, lens, offs); break;
-    case 3: batch_one<i64, 1, PASS>(J, t, lens, offs); break;
-    case 4: batch_one<i64, 2, PASS>(J, t, lens, offs); break;
-    case 5: batch_one<i64, 3, PASS>(J, t, lens, offs); break;
-    case 6: batch_one<float, 1, PASS>(J, t, lens, offs); break;
-    case 7: batch_one<float, 2, PASS>(J, t, lens, offs); break;
-    case 8: batch_one<float, 3, PASS>(J, t, lens, offs); break;
-    case 9: batch_one<double, 1, PASS>(J, t, lens, offs); break;
-    case 10: batch_one<double, 2, PASS>(J, t, lens, offs); break;
-    default: batch_one<double, 3, PASS>(J, t, lens, offs); break;
+    case 0: batch_one<i32, 1, PASS>(J, t, lens, offs, slot); break;
+    case 1: batch_one<i32, 2, PASS>(J, t, lens, offs, slot); break;
+    case 2: batch_one<i32, 3, PASS>(J, t, lens, offs, slot); break;
+    case 3: batch_one<i64, 1, PASS>(J, t, lens, offs, slot); break;
+    case 4: batch_one<i64, 2, PASS>(J, t, lens, offs, slot); break;
+    case 5: batch_one<i64, 3, PASS>(J, t, lens, offs, slot); break;
+    case 6: batch_one<float, 1, PASS>(J, t, lens, offs, slot); break;
+    case 7: batch_one<float, 2, PASS>(J, t, lens, offs, slot); break;
+    case 8: batch_one<float, 3, PASS>(J, t, lens, offs, slot); break;
+    case 9: batch_one<double, 1, PASS>(J, t, lens, offs, slot); break;
+    case 10: batch_one<double, 2, PASS>(J, t, lens, offs, slot); break;
+    default: batch_one<double, 3, PASS>(J, t, lens, offs, slot); break;
   }
 }
 
-// persistent kernel of one pass and family; tasks[i] = {job, first lane}
+// persistent kernel of one pass and family; tasks[i] = {job, first lane}.
+// stride (DEC of the 1-3D family): LDS slot stride in words, 0 when no job
+// is staged; the launch gives blockDim.x * stride words of LDS.
 template <int PASS, bool WIDE>
 __global__ void __launch_bounds__(256, 2)
 k_batch(const BJob* __restrict__ jobs, const uint2* __restrict__ tasks, u32 ntasks, u32 per_task,
-        u64* __restrict__ lens, const u64* __restrict__ offs)
+        u64* __restrict__ lens, const u64* __restrict__ offs, u32 stride)
 {
+  extern __shared__ u64 kslots[];
   const u32 lane = threadIdx.x & 63;
   const u32 wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const u32 nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -165,9 +185,24 @@ k_batch(const BJob* __restrict__ jobs, const uint2* __restrict__ tasks, u32 ntas
     const BJob& J = jobs[tk.x];
     const u32 end = umin(tk.y + per_task, J.nblocks);
     for (u32 t0 = tk.y; t0 < end; t0 += 64) {
+      const u64* slot = nullptr;
+      if constexpr (PASS == BK_DEC && !WIDE) {
+        if (stride && J.stage) {
+          // the wave's blocks t0 .. t0 + cnt - 1 are one run of words; the
+          // previous chunk's slots are read before they are overwritten
+          const u32 nw = J.z.maxbits >> 6;
+          const u32 cnt = umin(64u, end - t0);
+          u64* wslots = kslots + (u64)(threadIdx.x - lane) * stride;
+          wave_lds_sync();
+          wave_copy<false, true>(J.words, J.nwords, wslots, stride, nw, cnt, (J.offset >> 6) + (u64)t0 * nw,
+                                 nullptr);
+          wave_lds_sync();
+          slot = wslots + (u64)lane * stride;
+        }
+      }
       const u32 t = t0 + lane;
       if (t < end)
-        batch_dispatch<PASS, WIDE>(J, t, lens, offs);
+        batch_dispatch<PASS, WIDE>(J, t, lens, offs, slot);
     }
   }
 }
@@ -177,10 +212,12 @@ k_batch(const BJob* __restrict__ jobs, const uint2* __restrict__ tasks, u32 ntas
 // batch_wide_{len,enc,dec}.hip) so that they compile in parallel.
 template <int PASS, bool WIDE>
 int launch_batch_pass(unsigned grid, unsigned wg, hipStream_t st, const BJob* jobs, const uint2* tasks,
-                      u32 ntasks, u32 per_task, u64* lens, const u64* offs)
+                      u32 ntasks, u32 per_task, u64* lens, const u64* offs, u32 stride)
 {
-  hipLaunchKernelGGL((k_batch<PASS, WIDE>), dim3(grid), dim3(wg), 0, st, jobs, tasks, ntasks, per_task, lens,
-                     offs);
+  if (PASS != BK_DEC || WIDE)
+    stride = 0;
+  hipLaunchKernelGGL((k_batch<PASS, WIDE>), dim3(grid), dim3(wg), (size_t)wg * stride * 8, st, jobs, tasks, ntasks,
+                     per_task, lens, offs, stride);
   HIPCHK(hipGetLastError());
   return ZFP_HIP_OK;
 }
@@ -199,7 +236,7 @@ int launch_batch_totals(hipStream_t st, const BJob* jobs, u32 njobs, const u64* 
 
 #define ZH_BATCH_PASS(P, W)                                                                                \
   int launch_batch_pass<P, W>(unsigned, unsigned, hipStream_t, const BJob*, const uint2*, u32, u32, u64*,       \
-                              const u64*)
+                              const u64*, u32)
 extern template ZH_BATCH_PASS(BK_LEN, false);
 extern template ZH_BATCH_PASS(BK_ENC, false);
 extern template ZH_BATCH_PASS(BK_DEC, false);
@@ -208,17 +245,29 @@ extern template ZH_BATCH_PASS(BK_ENC, true);
 extern template ZH_BATCH_PASS(BK_DEC, true);
 
 // both families' launches of one pass over task lists split by family
-// (tasks[0, nfast) fast, then nwide wide tasks)
+// (tasks[0, nfast) fast, then nwide wide tasks); stride: DEC's slot stride
+// for staged jobs (batch_stage_stride)
 template <int PASS>
 int launch_batch_families(unsigned grid, unsigned wg, hipStream_t st, const BJob* jobs, const uint2* tasks,
-                          u32 nfast, u32 nwide, u32 per_task, u64* lens, const u64* offs)
+                          u32 nfast, u32 nwide, u32 per_task, u64* lens, const u64* offs, u32 stride = 0)
 {
   int rc = ZFP_HIP_OK;
   if (nfast)
-    rc = launch_batch_pass<PASS, false>(grid, wg, st, jobs, tasks, nfast, per_task, lens, offs);
+    rc = launch_batch_pass<PASS, false>(grid, wg, st, jobs, tasks, nfast, per_task, lens, offs, stride);
   if (!rc && nwide)
-    rc = launch_batch_pass<PASS, true>(grid, wg, st, jobs, tasks + nfast, nwide, per_task, lens, offs);
+    rc = launch_batch_pass<PASS, true>(grid, wg, st, jobs, tasks + nfast, nwide, per_task, lens, offs, 0);
   return rc;
+}
+
+// DEC's LDS slot stride for the staged jobs among jobs[0, n): odd, at least
+// one word past the longest block; 0 when none is staged
+inline u32 batch_stage_stride(const BJob* jobs, size_t n)
+{
+  u32 nw = 0;
+  for (size_t j = 0; j < n; j++)
+    if (jobs[j].stage)
+      nw = std::max(nw, jobs[j].z.maxbits >> 6);
+  return nw ? (nw + 1) | 1u : 0u;
 }
 
 } // namespace zfphip

@@ -315,7 +315,7 @@ struct Server {
                                          s.d_offs.p);
     if (!rc)
       rc = launch_batch_families<BK_DEC>(grid, SRV_WG, s.st, s.d_jobs.p, t_dec, nd0, nd1, SRV_BPT, s.d_lens.p,
-                                         s.d_offs.p);
+                                         s.d_offs.p, batch_stage_stride(s.h_jobs.p, n));
     if (!rc)
       rc = launch_batch_totals(s.st, s.d_jobs.p, (u32)n, s.d_offs.p, s.d_bits.p);
     if (rc)
