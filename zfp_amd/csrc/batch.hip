@@ -201,18 +201,19 @@ struct Batch {
     if (dir == ZFP_HIP_BATCH_COMPRESS) {
       if (nvar) {
         rc = launch_batch_families<BK_LEN>(grid, s.wg, st, d_jobs, d_tasks + na0 + na1, nv0, nv1, s.bpt, d_lens,
-                                           d_offs);
+                                           d_offs, 0);
         if (!rc)
           rc = exclusive_scan_tmp(d_lens, d_offs, nvar + 1, d_tmp, st);
       }
       if (!rc && !prep.empty())
         rc = launch_batch_prepare(ncu * 4, st, d_jobs, d_prep, (u32)prep.size(), d_offs);
       if (!rc)
-        rc = launch_batch_families<BK_ENC>(grid, s.wg, st, d_jobs, d_tasks, na0, na1, s.bpt, d_lens, d_offs);
+        rc = launch_batch_families<BK_ENC>(grid, s.wg, st, d_jobs, d_tasks, na0, na1, s.bpt, d_lens, d_offs,
+                                           batch_stage_stride(jobs.data(), jobs.size(), 0));
     }
     else {
       rc = launch_batch_families<BK_DEC>(grid, s.wg, st, d_jobs, d_tasks, na0, na1, s.bpt, d_lens, d_offs,
-                                         batch_stage_stride(jobs.data(), jobs.size()));
+                                         batch_stage_stride(jobs.data(), jobs.size(), 1));
     }
     if (!rc)
       rc = launch_batch_totals(st, d_jobs, njobs, d_offs, d_bits);
@@ -336,15 +337,15 @@ int job_to_device(const zfp_hip_job& in, int dir, BJob& J, u64& vbase)
     // is known, which bounds a zero capacity too
     const u64 need = (in.offset + (u64)d.nblocks * in.maxbits + 63) / 64;
     J.nwords = in.nwords ? in.nwords : need;
-    // fixed rate, word-aligned blocks in stream order: staged through LDS
-    bool order = true;
-    for (int i = 0; i < d.dims; i++)
-      order = order && d.g.lax[i] == (u32)i;
-    J.stage = J.mode == BM_FIXED && !J.wide && order && !(in.offset & 63) && !(in.maxbits & 63) &&
-                      in.maxbits / 64 <= BATCH_STAGE_WORDS && J.nwords > 0
-                  ? 1u
-                  : 0u;
   }
+  // fixed rate, word-aligned blocks in stream order: staged through LDS
+  bool order = true;
+  for (int i = 0; i < d.dims; i++)
+    order = order && d.g.lax[i] == (u32)i;
+  J.stage = J.mode == BM_FIXED && !J.wide && order && !(in.offset & 63) && !(in.maxbits & 63) &&
+                    in.maxbits / 64 <= BATCH_STAGE_WORDS && (!J.dir || J.nwords > 0)
+                ? 1u
+                : 0u;
   return ZFP_HIP_OK;
 }
 

@@ -19,10 +19,12 @@
 // Passes: LEN (block lengths of variable-rate jobs), then an exclusive scan
 // of those lengths, ENC (every job writes its blocks: fixed rate at
 // b * maxbits, variable rate at its scanned offset) and, for decompression,
-// DEC.  DEC stages the blocks of fixed-rate jobs with word-aligned blocks in
-// stream order through LDS, one slot per lane, as the single-field decoder
-// does: the wave's 64 blocks are one run of stream words, copied with
-// coalesced loads, and the decoder then reads its slot by bit index.
+// DEC.  ENC and DEC stage the blocks of fixed-rate jobs with word-aligned
+// blocks in stream order through LDS, one slot per lane, as the single-field
+// kernels do: the wave's 64 blocks are one run of stream words, which ENC
+// codes into the slots (the straight-line slot coder) and then stores with
+// coalesced stores, and DEC copies in with coalesced loads and decodes from
+// the slots by bit index.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -55,13 +57,13 @@ struct BJob {
   u32 dims;      // 1-4
   u32 wide;      // family: 0 for 1-3D lossy, 1 for 4D or reversible
   u32 rev;       // reversible mode
-  u32 stage;     // DEC stages the blocks through LDS slots (fixed rate,
-                 // word-aligned blocks of at most BATCH_STAGE_WORDS words,
-                 // lanes in stream order)
+  u32 stage;     // ENC or DEC stages the blocks through LDS slots (fixed
+                 // rate, word-aligned blocks of at most BATCH_STAGE_WORDS
+                 // words, lanes in stream order, 1-3D lossy)
 };
 
-// the longest blocks, in words, that DEC stages through LDS: a workgroup of
-// 256 lanes then needs at most 256 x 17 words (34 KiB)
+// the longest blocks, in words, that ENC and DEC stage through LDS: a
+// workgroup of 256 lanes then needs at most 256 x 19 words (38 KiB)
 constexpr u32 BATCH_STAGE_WORDS = 16;
 
 // the kind of a job within its family
@@ -72,9 +74,11 @@ inline u32 job_kind(u32 type, u32 dims, bool rev)
   return rev ? 4u + 4u * (type - 1) + (dims - 1) : type - 1;
 }
 
-// slot: the lane's LDS slot holding its block (staged DEC jobs), else null
+// slot: the lane's LDS slot (staged jobs): ENC codes the block into it
+// (zeroed, BATCH_STAGE_WORDS + 2 words at most), DEC reads the block from
+// it; null otherwise
 template <typename S, int D, int PASS, bool REV = false>
-__device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const u64* offs, const u64* slot)
+__device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const u64* offs, u64* slot)
 {
   u32 bc[4] = {0, 0, 0, 0};
   const u32 b = lane_block<D>(t, J.g, bc);
@@ -87,6 +91,12 @@ __device__ __forceinline__ void batch_one(const BJob& J, u32 t, u64* lens, const
       lens[J.vbase + b] = encode_block<S, D>(c, (const S*)J.data, J.g, bc, J.z);
   }
   else if constexpr (PASS == BK_ENC) {
+    if (!REV && slot) {
+      SlotWriter w;
+      w.begin(slot, (J.z.maxbits >> 6) + 2);
+      encode_block<S, D>(w, (const S*)J.data, J.g, bc, J.z);
+      return;
+    }
     u64 s0, s1;
     if (J.mode == BM_FIXED) {
       s0 = (u64)b * J.z.maxbits;
@@ -141,7 +151,7 @@ __device__ __forceinline__ void batch_wide_type(const BJob& J, u32 t, u64* lens,
 }
 
 template <int PASS, bool WIDE>
-__device__ __forceinline__ void batch_dispatch(const BJob& J, u32 t, u64* lens, const u64* offs, const u64* slot)
+__device__ __forceinline__ void batch_dispatch(const BJob& J, u32 t, u64* lens, const u64* offs, u64* slot)
 {
   if constexpr (WIDE) {
     switch (J.type) {
@@ -169,8 +179,8 @@ __device__ __forceinline__ void batch_dispatch(const BJob& J, u32 t, u64* lens, 
 }
 
 // persistent kernel of one pass and family; tasks[i] = {job, first lane}.
-// stride (DEC of the 1-3D family): LDS slot stride in words, 0 when no job
-// is staged; the launch gives blockDim.x * stride words of LDS.
+// stride (ENC and DEC of the 1-3D family): LDS slot stride in words, 0 when
+// no job is staged; the launch gives blockDim.x * stride words of LDS.
 template <int PASS, bool WIDE>
 __global__ void __launch_bounds__(256, 2)
 k_batch(const BJob* __restrict__ jobs, const uint2* __restrict__ tasks, u32 ntasks, u32 per_task,
@@ -184,25 +194,37 @@ k_batch(const BJob* __restrict__ jobs, const uint2* __restrict__ tasks, u32 ntas
     const uint2 tk = tasks[task];
     const BJob& J = jobs[tk.x];
     const u32 end = umin(tk.y + per_task, J.nblocks);
+    // staged job: the wave's blocks t0 .. t0 + cnt - 1 of each chunk are
+    // one run of stream words from (offset / 64) + t0 * nw
+    const bool staged = PASS != BK_LEN && !WIDE && stride && J.stage;
+    const u32 nw = J.z.maxbits >> 6;
+    u64* wslots = kslots + (u64)(threadIdx.x - lane) * stride;
     for (u32 t0 = tk.y; t0 < end; t0 += 64) {
-      const u64* slot = nullptr;
-      if constexpr (PASS == BK_DEC && !WIDE) {
-        if (stride && J.stage) {
-          // the wave's blocks t0 .. t0 + cnt - 1 are one run of words; the
-          // previous chunk's slots are read before they are overwritten
-          const u32 nw = J.z.maxbits >> 6;
-          const u32 cnt = umin(64u, end - t0);
-          u64* wslots = kslots + (u64)(threadIdx.x - lane) * stride;
-          wave_lds_sync();
+      u64* slot = nullptr;
+      const u32 cnt = umin(64u, end - t0);
+      if (staged) {
+        // the previous chunk's slots are read before they are overwritten
+        wave_lds_sync();
+        slot = wslots + (u64)lane * stride;
+        if constexpr (PASS == BK_DEC) {
           wave_copy<false, true>(J.words, J.nwords, wslots, stride, nw, cnt, (J.offset >> 6) + (u64)t0 * nw,
                                  nullptr);
           wave_lds_sync();
-          slot = wslots + (u64)lane * stride;
+        }
+        else {
+          for (u32 i = 0; i < nw + 2; i++)
+            slot[i] = 0;
         }
       }
       const u32 t = t0 + lane;
       if (t < end)
         batch_dispatch<PASS, WIDE>(J, t, lens, offs, slot);
+      if constexpr (PASS == BK_ENC) {
+        if (staged) {
+          wave_lds_sync();
+          wave_copy<true, true>(J.words, 0, wslots, stride, nw, cnt, (J.offset >> 6) + (u64)t0 * nw, nullptr);
+        }
+      }
     }
   }
 }
@@ -214,7 +236,7 @@ template <int PASS, bool WIDE>
 int launch_batch_pass(unsigned grid, unsigned wg, hipStream_t st, const BJob* jobs, const uint2* tasks,
                       u32 ntasks, u32 per_task, u64* lens, const u64* offs, u32 stride)
 {
-  if (PASS != BK_DEC || WIDE)
+  if (PASS == BK_LEN || WIDE)
     stride = 0;
   hipLaunchKernelGGL((k_batch<PASS, WIDE>), dim3(grid), dim3(wg), (size_t)wg * stride * 8, st, jobs, tasks, ntasks,
                      per_task, lens, offs, stride);
@@ -249,7 +271,7 @@ extern template ZH_BATCH_PASS(BK_DEC, true);
 // for staged jobs (batch_stage_stride)
 template <int PASS>
 int launch_batch_families(unsigned grid, unsigned wg, hipStream_t st, const BJob* jobs, const uint2* tasks,
-                          u32 nfast, u32 nwide, u32 per_task, u64* lens, const u64* offs, u32 stride = 0)
+                          u32 nfast, u32 nwide, u32 per_task, u64* lens, const u64* offs, u32 stride)
 {
   int rc = ZFP_HIP_OK;
   if (nfast)
@@ -259,15 +281,16 @@ int launch_batch_families(unsigned grid, unsigned wg, hipStream_t st, const BJob
   return rc;
 }
 
-// DEC's LDS slot stride for the staged jobs among jobs[0, n): odd, at least
-// one word past the longest block; 0 when none is staged
-inline u32 batch_stage_stride(const BJob* jobs, size_t n)
+// The LDS slot stride for the staged jobs among jobs[0, n) of one
+// direction: odd, and past the longest block by one word (DEC: the reader's
+// lookahead) or two (ENC: the slot coder's overflow); 0 when none is staged
+inline u32 batch_stage_stride(const BJob* jobs, size_t n, int dir)
 {
   u32 nw = 0;
   for (size_t j = 0; j < n; j++)
-    if (jobs[j].stage)
+    if (jobs[j].stage && (int)jobs[j].dir == dir)
       nw = std::max(nw, jobs[j].z.maxbits >> 6);
-  return nw ? (nw + 1) | 1u : 0u;
+  return nw ? (nw + (dir ? 1u : 2u)) | 1u : 0u;
 }
 
 } // namespace zfphip

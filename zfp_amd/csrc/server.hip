@@ -304,7 +304,7 @@ struct Server {
     if (vbase) {
       HIPCHK(hipMemsetAsync(s.d_lens.p + vbase, 0, sizeof(u64), s.st));
       rc = launch_batch_families<BK_LEN>(grid, SRV_WG, s.st, s.d_jobs.p, t_var, nv0, nv1, SRV_BPT, s.d_lens.p,
-                                         s.d_offs.p);
+                                         s.d_offs.p, 0);
       if (!rc)
         rc = exclusive_scan_tmp(s.d_lens.p, s.d_offs.p, vbase + 1, s.d_tmp.p, s.st);
     }
@@ -312,10 +312,10 @@ struct Server {
       rc = launch_batch_prepare(ncu * 4, s.st, s.d_jobs.p, s.d_prep.p, (u32)prep.size(), s.d_offs.p);
     if (!rc)
       rc = launch_batch_families<BK_ENC>(grid, SRV_WG, s.st, s.d_jobs.p, t_comp, nc0, nc1, SRV_BPT, s.d_lens.p,
-                                         s.d_offs.p);
+                                         s.d_offs.p, batch_stage_stride(s.h_jobs.p, n, 0));
     if (!rc)
       rc = launch_batch_families<BK_DEC>(grid, SRV_WG, s.st, s.d_jobs.p, t_dec, nd0, nd1, SRV_BPT, s.d_lens.p,
-                                         s.d_offs.p, batch_stage_stride(s.h_jobs.p, n));
+                                         s.d_offs.p, batch_stage_stride(s.h_jobs.p, n, 1));
     if (!rc)
       rc = launch_batch_totals(s.st, s.d_jobs.p, (u32)n, s.d_offs.p, s.d_bits.p);
     if (rc)
