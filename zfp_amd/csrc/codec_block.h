@@ -216,13 +216,15 @@ __device__ __forceinline__ u32 encode_block4(W& w, const S* data, const Geom& g,
   for (int i = 0; i < 64; i++)
     fwd_lift(q[i], q[64 + i], q[128 + i], q[192 + i]);
   u64 X[4 * T::P];
+  // planes 0-31 of 64-bit coefficients only when the block codes any of them
+  const bool low = T::P == 32 || maxprec > 32;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     UInt u[64];
 #pragma unroll
     for (int i = 0; i < 64; i++)
       u[i] = ((UInt)q[order_at<4>(64 * c + i)] + NB<UInt>::mask) ^ NB<UInt>::mask;
-    planes_chunk<UInt, T::P>(u, X, c);
+    planes_chunk<UInt, T::P>(u, X, c, low);
   }
   u32 c = code_planes256<T::P>(w, X, maxbits, maxprec);
   if (c < minbits) {
@@ -256,7 +258,9 @@ __device__ __forceinline__ u32 decode_block4(R& r, S* data, const Geom& g, const
     maxbits = z.maxbits - hdr;
   }
   u64 X[4 * T::P];
-  u32 c = decode_table256<T::P>(r, X, maxbits, maxprec);
+  // planes 0-31 of 64-bit coefficients only when the block codes any of them
+  const bool low = T::P == 32 || maxprec > 32;
+  u32 c = decode_table256<T::P>(r, X, maxbits, maxprec, low);
   if (c < minbits)
     c = minbits;
   if (STORE) {
@@ -264,7 +268,7 @@ __device__ __forceinline__ u32 decode_block4(R& r, S* data, const Geom& g, const
 #pragma unroll
     for (int ch = 0; ch < 4; ch++) {
       UInt u[64];
-      coeffs_chunk<UInt, T::P>(X, ch, u);
+      coeffs_chunk<UInt, T::P>(X, ch, u, low);
 #pragma unroll
       for (int i = 0; i < 64; i++)
         q[order_at<4>(64 * ch + i)] = (Int)((u[i] ^ NB<UInt>::mask) - NB<UInt>::mask);

@@ -1176,15 +1176,19 @@ struct PlaneDecoder {
 // coefficients 64c .. 64c+63 (bit i = bit k of coefficient 64c + i).  Built
 // with the 32x32 transposes of planes_half, one 64-coefficient chunk and
 // one half at a time, so that the coder reads four words per plane instead
-// of extracting a bit from each of the 256 coefficients.
+// of extracting a bit from each of the 256 coefficients.  low (64-bit
+// coefficients): build planes 0-31 too; a block whose lowest coded plane is
+// 32 or above leaves them out (and never reads them).
 template <typename UInt, int P>
-ZH_INLINE void planes_chunk(const UInt* u, u64* X, int c)
+ZH_INLINE void planes_chunk(const UInt* u, u64* X, int c, bool low = true)
 {
   PlaneVec lo, hi;
-  planes_half<64, 0>(u, lo, hi);
+  if (P == 32 || low) {
+    planes_half<64, 0>(u, lo, hi);
 #pragma unroll
-  for (int k = 0; k < 32; k++)
-    X[4 * k + c] = (u64)lo[k] | ((u64)hi[k] << 32);
+    for (int k = 0; k < 32; k++)
+      X[4 * k + c] = (u64)lo[k] | ((u64)hi[k] << 32);
+  }
   if constexpr (P == 64) {
     planes_half<64, 1>(u, lo, hi);
 #pragma unroll
@@ -1201,15 +1205,19 @@ ZH_INLINE void planes256(const UInt* u, u64* X)
     planes_chunk<UInt, P>(u + 64 * c, X, c);
 }
 
-// inverse of planes_chunk: coefficients 64c .. 64c+63 from the plane table
+// inverse of planes_chunk: coefficients 64c .. 64c+63 from the plane table;
+// without `low` (64-bit coefficients) planes 0-31 are taken as zero and not
+// read
 template <typename UInt, int P>
-ZH_INLINE void coeffs_chunk(const u64* X, int c, UInt* u)
+ZH_INLINE void coeffs_chunk(const u64* X, int c, UInt* u, bool low = true)
 {
 #pragma unroll
   for (int i = 0; i < 64; i++)
     u[i] = 0;
 #pragma unroll
   for (int H = 0; H < P / 32; H++) {
+    if (P == 64 && H == 0 && !low)
+      continue;
     PlaneVec lo, hi;
 #pragma unroll
     for (int k = 0; k < 32; k++) {
@@ -1299,16 +1307,17 @@ ZH_INLINE u32 encode_planes256(W& w, const UInt* u, u32 maxbits, u32 maxprec)
 }
 
 // the decoder into a plane table (planes_chunk layout); planes that are not
-// coded stay zero.  Returns the bits consumed.
+// coded stay zero (without `low`, planes 32 and up only: the caller does not
+// read the others).  Returns the bits consumed.
 template <int P, typename R>
-ZH_INLINE u32 decode_table256(R& r, u64* X, u32 maxbits, u32 maxprec)
+ZH_INLINE u32 decode_table256(R& r, u64* X, u32 maxbits, u32 maxprec, bool low = true)
 {
   const u32 N = 256;
   const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
   u32 budget = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
   const u32 budget0 = budget;
   u32 n = 0;
-  for (int i = 0; i < 4 * P; i++)
+  for (int i = P == 64 && !low ? 128 : 0; i < 4 * P; i++)
     X[i] = 0;
   for (int k = P - 1; k >= (int)kmin && budget; k--) {
     u64 x[4] = {0, 0, 0, 0};
