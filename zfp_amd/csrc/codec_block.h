@@ -226,7 +226,14 @@ __device__ __forceinline__ u32 encode_block4(W& w, const S* data, const Geom& g,
       u[i] = ((UInt)q[order_at<4>(64 * c + i)] + NB<UInt>::mask) ^ NB<UInt>::mask;
     planes_chunk<UInt, T::P>(u, X, c, low);
   }
-  u32 c = code_planes256<T::P>(w, X, maxbits, maxprec);
+  u32 c;
+  if constexpr (std::is_same<W, BitCounter>::value) {
+    // a length pass: the code's length in closed form per plane
+    c = count_planes256<T::P>(X, maxbits, maxprec);
+    w.zeros(c);
+  }
+  else
+    c = code_planes256<T::P>(w, X, maxbits, maxprec);
   if (c < minbits) {
     w.zeros(minbits - c);
     c = minbits;

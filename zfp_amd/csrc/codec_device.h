@@ -1298,6 +1298,45 @@ ZH_INLINE u32 code_planes256(W& w, const u64* X, u32 maxbits, u32 maxprec)
   return budget0 - budget;
 }
 
+// The length of code_planes256's code without writing it: per plane, the
+// verbatim bits of the n significant coefficients, then (n < 256) the group
+// tests of the rest: one 0 when none has a one-bit, else, with m one-bits
+// and the highest at coefficient h, h - n + m + 2 bits (h - n + m when h is
+// the last coefficient, whose one-bit is implied); cut to the budget, of
+// which code_planes256's code is a prefix.
+template <int P>
+ZH_INLINE u32 count_planes256(const u64* X, u32 maxbits, u32 maxprec)
+{
+  const u32 N = 256;
+  const u32 kmin = P > (int)maxprec ? P - maxprec : 0;
+  const u32 limit = budget_binds(maxbits, maxprec, N) ? maxbits : 0xffffffffu;
+  u32 pos = 0, n = 0;
+  for (int k = P - 1; k >= (int)kmin && pos < limit; k--) {
+    if (n >= N) {
+      pos += N;
+      continue;
+    }
+    u32 m = 0, h = 0;
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const u32 lo = 64u * (u32)c;
+      const u64 mask = n <= lo ? ~0ull : n >= lo + 64 ? 0ull : ~0ull << (n - lo);
+      const u64 v = X[4 * k + c] & mask;
+      m += (u32)__builtin_popcountll(v);
+      h = v ? lo + 63u - (u32)__builtin_clzll(v) : h;
+      any = any || v != 0;
+    }
+    if (!any)
+      pos += n + 1;
+    else {
+      pos += n + (h - n) + m + (h == N - 1 ? 0u : 2u);
+      n = h + 1;
+    }
+  }
+  return umin(pos, limit);
+}
+
 template <typename UInt, int P, typename W>
 ZH_INLINE u32 encode_planes256(W& w, const UInt* u, u32 maxbits, u32 maxprec)
 {
