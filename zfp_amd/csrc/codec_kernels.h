@@ -336,6 +336,80 @@ k_encode_waves(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* _
   }
 }
 
+// Variable-length coding of long blocks (more than 32 words, beyond the LDS
+// slots of k_encode_stage) with lanes in stream order, every block coded
+// once:
+//
+//   k_encode_blocks  codes block b into its own bw words of `stage` (from
+//                    bit 0, plain stores) and writes its length;
+//   exclusive scan   of the lengths: the block offsets;
+//   k_place_blocks   k_encode_waves with the staged bits in place of a
+//                    second coding: each wave assembles its blocks' bits in
+//                    LDS and stores whole words, OR-ing only its two edge
+//                    words (zeroed by k_prepare).
+template <typename S, int D, bool REV>
+__global__ void __launch_bounds__(256)
+k_encode_blocks(const S* __restrict__ data, Geom g, u32 nblocks, Params z, u64* __restrict__ stage,
+                u64* __restrict__ lens, u32 bw)
+{
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblocks)
+    return;
+  u32 bc[4] = {0, 0, 0, 0};
+  const u32 b = lane_block<D>(t, g, bc); // == t: stream order
+  const u64 lo = (u64)b * bw * 64;
+  BitWriter<true> w;
+  w.begin(stage, lo, lo + (u64)bw * 64);
+  lens[b] = encode_any<S, D, REV>(w, data, g, bc, z);
+  w.finish();
+  if (t == 0)
+    lens[nblocks] = 0; // so that the exclusive scan's last entry is the total
+}
+
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(256)
+k_place_blocks(const u64* __restrict__ stage, u32 bw, u32 nblocks, u64* __restrict__ out, u64 offset,
+               const u64* __restrict__ boffs, u32 wcap)
+{
+  extern __shared__ u64 wstage[];
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 lane = threadIdx.x & 63u;
+  const u32 t0 = t - lane;
+  if (t0 >= nblocks)
+    return; // the whole wave lies past the field
+  const u32 tl = umin(t0 + 63u, nblocks - 1u);
+  const u64 first = offset + boffs[t0], end = offset + boffs[tl + 1];
+  const u64 wlo = first >> 6, whi = (end + 63) >> 6;
+  const bool staged = whi - wlo <= wcap;
+  const u32 nw = staged ? (u32)(whi - wlo) : 0u;
+  u64* st = wstage + (u64)(threadIdx.x >> 6) * wcap;
+  for (u32 i = lane; i < nw; i += 64)
+    st[i] = 0;
+  wave_lds_sync();
+  if (t < nblocks) {
+    WaveWriter w;
+    w.out = out; w.st = st; w.wbase = wlo; w.staged = staged;
+    w.lo = offset + boffs[t]; w.hi = offset + boffs[t + 1];
+    w.wi = w.lo >> 6; w.fill = (u32)(w.lo & 63); w.acc = 0;
+    const u64 len = boffs[t + 1] - boffs[t];
+    const u64* src = stage + (u64)t * bw;
+    for (u64 i = 0; 64 * i < len; i++) {
+      const u32 n = (u32)(len - 64 * i < 64 ? len - 64 * i : 64);
+      w.put(src[i] & lowmask(n), n);
+    }
+    w.finish();
+  }
+  wave_lds_sync();
+  for (u32 i = lane; i < nw; i += 64) {
+    const u64 v = st[i];
+    const bool edge = (i == 0 && (first & 63)) || (i + 1 == nw && (end & 63));
+    if (!edge)
+      out[wlo + i] = v;
+    else if (v)
+      atomicOr((unsigned long long*)(out + wlo + i), (unsigned long long)v);
+  }
+}
+
 // Bits one block can take in the stream's mode: the per-block term of
 // zfp_stream_maximum_size (src/zfp.c:711-742).
 template <typename S, int D, bool REV>
@@ -943,8 +1017,21 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
     if (rc != ZFP_HIP_ENOMEM)
       return rc;
   }
-  hipLaunchKernelGGL((k_encode<S, D, ENC_LENGTH, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
-                     d.nblocks, z, out, offset, (const u64*)nullptr, lens);
+  // longer blocks in stream order: each coded once into a staging stride
+  // of bw words when those fit in device memory (k_encode_blocks), else
+  // a length pass now and a second coding in the write pass
+  u64* bstage = nullptr;
+  if (stream_order && !ws && bw > 32 && d.nblocks > 0) {
+    void* tp;
+    if (g_stage.get((u64)d.nblocks * bw * sizeof(u64), &tp) == ZFP_HIP_OK)
+      bstage = (u64*)tp;
+  }
+  if (bstage)
+    hipLaunchKernelGGL((k_encode_blocks<S, D, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g, d.nblocks,
+                       z, bstage, lens, bw);
+  else
+    hipLaunchKernelGGL((k_encode<S, D, ENC_LENGTH, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
+                       d.nblocks, z, out, offset, (const u64*)nullptr, lens);
   HIPCHK(hipGetLastError());
   rc = ws ? exclusive_scan_tmp(lens, offs, nb1, ws->tmp, st) : exclusive_scan(lens, offs, nb1, st);
   if (rc)
@@ -982,7 +1069,11 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
     hipLaunchKernelGGL(k_prepare_dev, dim3(std::min(grid, 2048u)), dim3(256), 0, st, out, offset,
                        (const u64*)(offs + d.nblocks));
   }
-  if (stream_order) {
+  if (bstage) {
+    hipLaunchKernelGGL(k_place_blocks<0>, dim3(grid), dim3(BS), (size_t)4 * wcap * 8, st, (const u64*)bstage, bw,
+                       d.nblocks, out, offset, (const u64*)offs, wcap);
+  }
+  else if (stream_order) {
     hipLaunchKernelGGL((k_encode_waves<S, D, REV>), dim3(grid), dim3(BS), (size_t)4 * wcap * 8, st, (const S*)data,
                        d.g, d.nblocks, z, out, offset, (const u64*)offs, wcap);
   }
