@@ -1037,13 +1037,16 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
   if (rc)
     return rc;
   // a wave's LDS area in the write pass: twice the words 64 average blocks
-  // take, at least 64 and at most 2048 (16 KiB).  A wave whose range is
-  // longer writes directly, so the size sets the speed, never the stream.
-  // It is taken from the last encode with the same codec parameters; only
-  // the first such encode waits for the stream length on the host.
+  // take (64 blocks of b bits take b words), at least 64 and at most 8192
+  // (64 KiB); k_encode_waves, four waves per workgroup, takes at most 2048
+  // of them, k_place_blocks, one wave per workgroup, all.  A wave whose
+  // range is longer writes directly, so the size sets the speed, never the
+  // stream.  It is taken from the last encode with the same codec
+  // parameters; only the first such encode waits for the stream length on
+  // the host.
   auto wcap_for = [&](u64 bits) {
-    const u64 avg_words = bits / (d.nblocks ? d.nblocks : 1) + 1;
-    return (u32)std::min<u64>(2048, std::max<u64>(64, 2 * avg_words + 4));
+    const u64 avg_bits = bits / (d.nblocks ? d.nblocks : 1) + 1;
+    return (u32)std::min<u64>(8192, std::max<u64>(64, 2 * avg_bits + 4));
   };
   // (The workspace variant always waits: it keeps no host-side state.)
   static std::map<std::tuple<u32, u32, u32, int>, u32> last_wcap;
@@ -1070,12 +1073,13 @@ int launch_encode_var(const Desc& d, const void* data, const Params& z, u64* out
                        (const u64*)(offs + d.nblocks));
   }
   if (bstage) {
-    hipLaunchKernelGGL(k_place_blocks<0>, dim3(grid), dim3(BS), (size_t)4 * wcap * 8, st, (const u64*)bstage, bw,
-                       d.nblocks, out, offset, (const u64*)offs, wcap);
+    hipLaunchKernelGGL(k_place_blocks<0>, dim3(grid_for(d.nblocks, 64)), dim3(64), (size_t)wcap * 8, st,
+                       (const u64*)bstage, bw, d.nblocks, out, offset, (const u64*)offs, wcap);
   }
   else if (stream_order) {
-    hipLaunchKernelGGL((k_encode_waves<S, D, REV>), dim3(grid), dim3(BS), (size_t)4 * wcap * 8, st, (const S*)data,
-                       d.g, d.nblocks, z, out, offset, (const u64*)offs, wcap);
+    const u32 wc = std::min<u32>(wcap, 2048);
+    hipLaunchKernelGGL((k_encode_waves<S, D, REV>), dim3(grid), dim3(BS), (size_t)4 * wc * 8, st, (const S*)data,
+                       d.g, d.nblocks, z, out, offset, (const u64*)offs, wc);
   }
   else {
     hipLaunchKernelGGL((k_encode<S, D, ENC_OFFSETS, REV>), dim3(grid), dim3(BS), 0, st, (const S*)data, d.g,
