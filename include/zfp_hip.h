@@ -164,6 +164,9 @@ int zfp_hip_decode_blocks_host(int type, unsigned dims, size_t count, void* bloc
 size_t zfp_hip_index_size(const void* stream_begin);
 int zfp_hip_index_export(const void* stream_begin, uint64_t* offsets, size_t count);
 int zfp_hip_index_import(const void* stream_begin, const uint64_t* offsets, size_t count);
+/* Frees every recorded index and the codec's device scratch (block lengths,
+   scan tiles, the staged bits of variable-rate compression); the next call
+   that needs scratch allocates it again. */
 void zfp_hip_index_clear(void);
 
 /* The same index in the reference's hybrid4 layout

@@ -84,6 +84,13 @@ struct Scratch {
     *out = p;
     return ZFP_HIP_OK;
   }
+  void release()
+  {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
 };
 
 // scratch of the variable-rate passes (lengths + offsets, scan tiles), the

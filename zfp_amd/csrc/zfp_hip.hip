@@ -1098,4 +1098,7 @@ extern "C" void zfp_hip_index_clear(void)
     (void)hipFree(g_spare);
   g_spare = nullptr;
   g_spare_bytes = 0;
+  // the codec's grow-only scratch (lengths, scan tiles, flags, staged bits)
+  for (Scratch* sc : {&g_lens, &g_scan_tmp, &g_flag, &g_stage})
+    sc->release();
 }
