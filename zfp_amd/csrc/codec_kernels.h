@@ -393,9 +393,21 @@ k_place_blocks(const u64* __restrict__ stage, u32 bw, u32 nblocks, u64* __restri
     w.wi = w.lo >> 6; w.fill = (u32)(w.lo & 63); w.acc = 0;
     const u64 len = boffs[t + 1] - boffs[t];
     const u64* src = stage + (u64)t * bw;
-    for (u64 i = 0; 64 * i < len; i++) {
-      const u32 n = (u32)(len - 64 * i < 64 ? len - 64 * i : 64);
-      w.put(src[i] & lowmask(n), n);
+    // eight loads in flight at a time (words of the block's own stride,
+    // always readable), then their bits in order
+    for (u64 i0 = 0; 64 * i0 < len; i0 += 8) {
+      u64 v[8];
+#pragma unroll
+      for (u32 j = 0; j < 8; j++)
+        v[j] = i0 + j < bw ? src[i0 + j] : 0ull;
+#pragma unroll
+      for (u32 j = 0; j < 8; j++) {
+        const u64 at = 64 * (i0 + j);
+        if (at < len) {
+          const u32 n = (u32)(len - at < 64 ? len - at : 64);
+          w.put(v[j] & lowmask(n), n);
+        }
+      }
     }
     w.finish();
   }
